@@ -1,0 +1,65 @@
+"""Compile the native libraries in-tree.
+
+* ``libh2p_host.so`` -- host kernel-block evaluation for the construction
+  (gcc, OpenMP).
+* ``libh2p_hip.so``  -- the MI355X engine behind ``include/h2p_hip.h``
+  (hipcc, gfx950).
+
+Both land in ``h2pack_amd/`` so they travel with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST_SO = os.path.join(PKG, "libh2p_host.so")
+HIP_SO = os.path.join(PKG, "libh2p_hip.so")
+
+HIP_SOURCES = ["h2p_hip.hip"]
+HIP_HEADERS = ["h2p_kernels.h", "h2p_device.h"]
+
+# Packed-FP32 instructions stay disabled: hipcc -O3 would otherwise form
+# v_pk_{add,mul,fma}_f32 from ordinary float code.
+HIPCC_FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-result"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    return r.stdout
+
+
+def build_host() -> str:
+    _run(["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared",
+          "-o", HOST_SO + ".tmp", os.path.join(CSRC, "h2p_host.c"), "-lm"])
+    os.replace(HOST_SO + ".tmp", HOST_SO)
+    return HOST_SO
+
+
+def build_hip() -> str:
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run(["/opt/rocm/bin/hipcc", *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
+          "-o", HIP_SO + ".tmp", *srcs])
+    os.replace(HIP_SO + ".tmp", HIP_SO)
+    return HIP_SO
+
+
+def build_all() -> None:
+    build_host()
+    build_hip()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["host", "hip"]
+    if "host" in what:
+        print(build_host())
+    if "hip" in what:
+        print(build_hip())

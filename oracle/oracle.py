@@ -1,0 +1,188 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes binding of ``oracle/_build/libh2oracle.so`` (the CPU restatement of the
+reference's H2P_matvec / H2P_matmul, see h2oracle.h).  Imported only by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker; the product path never reaches it.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "_build", "libh2oracle.so")
+
+_IP = ct.POINTER(ct.c_int)
+_DP = ct.POINTER(ct.c_double)
+
+
+class H2OH2Pack(ct.Structure):
+    _fields_ = [
+        ("pt_dim", ct.c_int), ("xpt_dim", ct.c_int), ("krnl_dim", ct.c_int),
+        ("n_point", ct.c_int), ("krnl_mat_size", ct.c_int),
+        ("n_node", ct.c_int), ("n_leaf_node", ct.c_int), ("max_child", ct.c_int),
+        ("max_level", ct.c_int), ("min_adm_level", ct.c_int),
+        ("children", ct.c_void_p), ("n_child", ct.c_void_p), ("node_level", ct.c_void_p),
+        ("pt_cluster", ct.c_void_p), ("mat_cluster", ct.c_void_p),
+        ("level_n_node", ct.c_void_p), ("level_nodes", ct.c_void_p), ("leaf_nodes", ct.c_void_p),
+        ("n_r_adm_pair", ct.c_int), ("n_r_inadm_pair", ct.c_int),
+        ("r_adm_pairs", ct.c_void_p), ("r_inadm_pairs", ct.c_void_p), ("node_n_r_adm", ct.c_void_p),
+        ("U_nrow", ct.c_void_p), ("U_ncol", ct.c_void_p), ("U", ct.c_void_p),
+        ("J_npt", ct.c_void_p), ("J_coord", ct.c_void_p),
+        ("coord", ct.c_void_p), ("fwd_pmt_idx", ct.c_void_p), ("bwd_pmt_idx", ct.c_void_p),
+        ("B_nrow", ct.c_void_p), ("B_ncol", ct.c_void_p), ("B_ptr", ct.c_void_p), ("B_data", ct.c_void_p),
+        ("D_nrow", ct.c_void_p), ("D_ncol", ct.c_void_p), ("D_ptr", ct.c_void_p), ("D_data", ct.c_void_p),
+        ("B_p2i_rowptr", ct.c_void_p), ("B_p2i_colidx", ct.c_void_p), ("B_p2i_val", ct.c_void_p),
+        ("D_p2i_rowptr", ct.c_void_p), ("D_p2i_colidx", ct.c_void_p), ("D_p2i_val", ct.c_void_p),
+        ("krnl_id", ct.c_int), ("krnl_param", ct.c_double * 4), ("n_krnl_param", ct.c_int),
+        ("BD_JIT", ct.c_int),
+    ]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return SO
+
+
+def lib() -> ct.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            build()
+        L = ct.CDLL(SO)
+        L.h2o_matvec.argtypes = [ct.POINTER(H2OH2Pack), ct.c_void_p, ct.c_void_p, ct.c_int]
+        L.h2o_matvec.restype = ct.c_int
+        L.h2o_matmul.argtypes = [ct.POINTER(H2OH2Pack), ct.c_int, ct.c_int, ct.c_void_p,
+                                 ct.c_int, ct.c_void_p, ct.c_int, ct.c_int]
+        L.h2o_matmul.restype = ct.c_int
+        L.h2o_kernel_eval.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
+                                      ct.c_int, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int]
+        L.h2o_kernel_eval.restype = None
+        L.h2o_direct_nbody.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p,
+                                       ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                       ct.c_int, ct.c_void_p, ct.c_int]
+        L.h2o_direct_nbody.restype = None
+        _lib = L
+    return _lib
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class OracleH2:
+    """Keeps the arrays alive and exposes the oracle's matvec/matmul on an
+    H2 representation produced by ``h2pack_amd.construct``."""
+
+    def __init__(self, h2, BD_JIT: int | None = None):
+        self.h2 = h2
+        jit = h2.BD_JIT if BD_JIT is None else int(BD_JIT)
+        if not jit and (h2.B_data is None or h2.D_data is None):
+            raise ValueError("AOT oracle needs host B/D blocks (build with host_blocks=True)")
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        s = H2OH2Pack()
+        s.pt_dim, s.xpt_dim, s.krnl_dim = h2.pt_dim, h2.xpt_dim, h2.krnl_dim
+        s.n_point, s.krnl_mat_size = h2.n_point, h2.krnl_mat_size
+        s.n_node, s.n_leaf_node, s.max_child = h2.n_node, h2.n_leaf_node, h2.max_child
+        s.max_level, s.min_adm_level = h2.max_level, h2.min_adm_level
+        for f in ("children", "n_child", "node_level", "pt_cluster", "mat_cluster",
+                  "level_n_node", "level_nodes", "r_adm_pairs", "r_inadm_pairs", "node_n_r_adm",
+                  "U_nrow", "U_ncol", "fwd_pmt_idx", "bwd_pmt_idx", "B_nrow", "B_ncol",
+                  "D_nrow", "D_ncol", "B_p2i_rowptr", "B_p2i_colidx", "B_p2i_val",
+                  "D_p2i_rowptr", "D_p2i_colidx", "D_p2i_val"):
+            setattr(s, f, arr(getattr(h2, f), np.int32))
+        s.leaf_nodes = arr(h2.leaf_nodes, np.int32)
+        s.n_r_adm_pair, s.n_r_inadm_pair = h2.n_r_adm_pair, h2.n_r_inadm_pair
+        Us = [np.ascontiguousarray(u, dtype=np.float64) for u in h2.U]
+        keep.extend(Us)
+        uptr = (ct.c_void_p * h2.n_node)(*[u.ctypes.data if u.size else None for u in Us])
+        keep.append(uptr)
+        s.U = ct.cast(uptr, ct.c_void_p)
+        jc = [np.ascontiguousarray(j, dtype=np.float64) for j in h2.J_coord]
+        keep.extend(jc)
+        jptr = (ct.c_void_p * h2.n_node)(*[j.ctypes.data if j.size else None for j in jc])
+        keep.append(jptr)
+        s.J_coord = ct.cast(jptr, ct.c_void_p)
+        s.J_npt = arr([j.size for j in h2.J], np.int32)
+        s.coord = arr(h2.coord, np.float64)
+        s.B_ptr = arr(h2.B_ptr, np.uint64)
+        s.D_ptr = arr(h2.D_ptr, np.uint64)
+        s.B_data = arr(h2.B_data, np.float64) if not jit else None
+        s.D_data = arr(h2.D_data, np.float64) if not jit else None
+        s.krnl_id = h2.krnl_id
+        p = list(h2.krnl_param)
+        for i, v in enumerate(p[:4]):
+            s.krnl_param[i] = float(v)
+        s.n_krnl_param = len(p)
+        s.BD_JIT = jit
+        self.s = s
+        self._keep = keep
+
+    def matvec(self, x: np.ndarray, n_thread: int = 0) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.h2.krnl_mat_size
+        y = np.empty_like(x)
+        rc = lib().h2o_matvec(ct.byref(self.s), x.ctypes.data, y.ctypes.data, n_thread)
+        if rc != 0:
+            raise MemoryError("oracle matvec allocation failed")
+        return y
+
+    def matmul(self, X: np.ndarray, layout: str = "col", n_thread: int = 0) -> np.ndarray:
+        """X is (N*kd, n_vec); layout selects how it is handed to the C code."""
+        N = self.h2.krnl_mat_size
+        n_vec = X.shape[1]
+        if layout == "row":
+            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            Y = np.empty_like(Xc)
+            rc = lib().h2o_matmul(ct.byref(self.s), 101, n_vec, Xc.ctypes.data, n_vec,
+                                  Y.ctypes.data, n_vec, n_thread)
+            out = Y
+        else:
+            Xc = np.asfortranarray(X, dtype=np.float64)
+            Y = np.empty((N, n_vec), order="F")
+            rc = lib().h2o_matmul(ct.byref(self.s), 102, n_vec, Xc.ctypes.data, N,
+                                  Y.ctypes.data, N, n_thread)
+            out = Y
+        if rc != 0:
+            raise MemoryError("oracle matmul allocation failed")
+        return np.ascontiguousarray(out)
+
+
+def kernel_eval(kid: int, param, c0: np.ndarray, c1: np.ndarray) -> np.ndarray:
+    kd = 3 if kid == 3 else 1
+    c0 = np.ascontiguousarray(c0, dtype=np.float64)
+    c1 = np.ascontiguousarray(c1, dtype=np.float64)
+    n0, n1 = c0.shape[1], c1.shape[1]
+    p = np.ascontiguousarray(list(param) + [0.0] * 4, dtype=np.float64)
+    out = np.empty((n0 * kd, n1 * kd))
+    lib().h2o_kernel_eval(kid, p.ctypes.data, len(param), c0.ctypes.data, n0, n0,
+                          c1.ctypes.data, n1, n1, out.ctypes.data, n1 * kd)
+    return out
+
+
+def direct_nbody(kid: int, param, coord: np.ndarray, x: np.ndarray, rows: np.ndarray,
+                 n_thread: int = 0) -> np.ndarray:
+    """y[rows] of the dense kernel product K(coord, coord) x (FP64)."""
+    kd = 3 if kid == 3 else 1
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    dst = np.ascontiguousarray(coord[:, rows])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(rows.size * kd)
+    p = np.ascontiguousarray(list(param) + [0.0] * 4, dtype=np.float64)
+    lib().h2o_direct_nbody(kid, p.ctypes.data, len(param), kd, coord.ctypes.data,
+                           coord.shape[1], coord.shape[1], x.ctypes.data,
+                           dst.ctypes.data, rows.size, rows.size, y.ctypes.data, n_thread)
+    return y
