@@ -1,0 +1,718 @@
+// MI355X H2 matvec / matmul engine: upload, schedules and the C ABI of
+// include/h2p_hip.h.
+//
+// Phase order of one matvec (src/H2Pack_matvec.c:1162-1284):
+//   gather x -> px                      (H2P_permute_vector_forward)
+//   upward sweep, one launch per level  (H2P_matvec_fwd_transform)
+//   ONE grouped launch of every B and D block, each block read or
+//   evaluated once, two partial outputs per block
+//                                       (intmd_mult_AOT/JIT + dense_mult_AOT/JIT)
+//   fixed-order reduction of partials into y1
+//   downward sweep, one launch per level (H2P_matvec_bwd_transform)
+//   leaf outputs: U y1 + partials, scattered to the caller's order
+//                                       (thread reduce + H2P_permute_vector_backward)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "h2p_hip.h"
+#include "h2p_kernels.h"
+
+using namespace h2p;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(H2P_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dev_upload(T **dptr, const std::vector<T> &v, size_t *acct) {
+    *dptr = nullptr;
+    size_t bytes = sizeof(T) * std::max<size_t>(v.size(), 1);
+    if (hipMalloc(dptr, bytes) != hipSuccess) return fail(H2P_HIP_ENOMEM, "hipMalloc failed");
+    *acct += bytes;
+    if (!v.empty()) HIP_TRY(hipMemcpy(*dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+int dev_alloc(T **dptr, size_t n, size_t *acct) {
+    *dptr = nullptr;
+    size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+    if (hipMalloc(dptr, bytes) != hipSuccess) return fail(H2P_HIP_ENOMEM, "hipMalloc failed");
+    *acct += bytes;
+    return 0;
+}
+
+struct Level {
+    int off, n;   // slice of a task array
+};
+
+inline bool fwdidx_ok(int v, int n) { return v >= 0 && v < n; }
+
+}  // namespace
+
+struct h2p_hip_handle {
+    int mode = 0, prec = 0, kid = 0, kd = 1, xpt_dim = 3;
+    int N = 0, n_point = 0, S = 0;
+    long long CT = 0;        // unified coordinate stride (points + skeleton points)
+    KParam kp{};
+    // device arrays
+    double *d_coord = nullptr;   // xpt_dim x CT
+    double *d_U = nullptr, *d_UT = nullptr;
+    double *d_blob = nullptr;    // AOT B then D
+    double *d_xin = nullptr;     // N + S
+    double *d_y1 = nullptr;      // S
+    double *d_arena = nullptr;
+    int *d_fwd = nullptr;
+    BlkTask *d_blk = nullptr;
+    FwdTask *d_fwdt = nullptr;
+    BwdTask *d_bwdt = nullptr;
+    RedTask *d_red = nullptr;
+    LeafTask *d_leaf = nullptr;
+    long long *d_contrib = nullptr;
+    double *d_mm_x = nullptr, *d_mm_y = nullptr;  // matmul staging
+    size_t mm_cap = 0;
+    std::vector<Level> fwd_levels, bwd_levels;
+    int n_blk = 0, n_red = 0, n_leaf = 0;
+    long long arena_len = 0;
+    size_t device_bytes = 0;
+    h2p_hip_info info{};
+    int profiling = 0;
+    hipEvent_t ev[7]{};
+    bool ev_ok = false;
+    hipStream_t own_stream = nullptr;
+};
+
+static void free_handle(h2p_hip_handle *h) {
+    if (!h) return;
+    void *ptrs[] = {h->d_coord, h->d_U, h->d_UT, h->d_blob, h->d_xin, h->d_y1, h->d_arena,
+                    h->d_fwd, h->d_blk, h->d_fwdt, h->d_bwdt, h->d_red, h->d_leaf,
+                    h->d_contrib, h->d_mm_x, h->d_mm_y};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->ev_ok)
+        for (auto &e : h->ev) (void)hipEventDestroy(e);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete h;
+}
+
+template <int KID, int KD>
+static void launch_jit(h2p_hip_handle *h, hipStream_t s) {
+    hipLaunchKernelGGL((k_blocks_jit<KID, KD>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                       h->d_coord, h->CT, h->kp, h->d_xin, h->d_arena);
+}
+
+template <int KID, int KD>
+static void launch_build(h2p_hip_handle *h, hipStream_t s) {
+    hipLaunchKernelGGL((k_build_blocks<KID, KD>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                       h->d_coord, h->CT, h->kp, h->d_blob);
+}
+
+static void dispatch_jit(h2p_hip_handle *h, hipStream_t s) {
+    switch (h->kid) {
+        case KC: launch_jit<KC, 1>(h, s); break;
+        case KG: launch_jit<KG, 1>(h, s); break;
+        case KM: launch_jit<KM, 1>(h, s); break;
+        default: launch_jit<KR, 3>(h, s); break;
+    }
+}
+
+static void dispatch_build(h2p_hip_handle *h, hipStream_t s) {
+    switch (h->kid) {
+        case KC: launch_build<KC, 1>(h, s); break;
+        case KG: launch_build<KG, 1>(h, s); break;
+        case KM: launch_build<KM, 1>(h, s); break;
+        default: launch_build<KR, 3>(h, s); break;
+    }
+}
+
+// One matvec on device pointers (x, y in the caller's order).
+static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
+    const bool prof = h->profiling && h->ev_ok;
+    if (prof) HIP_TRY(hipEventRecord(h->ev[0], s));
+    hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
+    for (const Level &L : h->fwd_levels)
+        hipLaunchKernelGGL(k_fwd, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
+    if (prof) HIP_TRY(hipEventRecord(h->ev[1], s));
+    if (h->n_blk > 0) {
+        if (h->mode == H2P_HIP_AOT)
+            hipLaunchKernelGGL(k_blocks_aot, dim3(h->n_blk), dim3(256), 0, s, h->d_blk, h->d_blob, h->d_xin, h->d_arena);
+        else
+            dispatch_jit(h, s);
+    }
+    if (prof) HIP_TRY(hipEventRecord(h->ev[2], s));
+    if (h->n_red > 0)
+        hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
+    if (prof) HIP_TRY(hipEventRecord(h->ev[3], s));
+    for (const Level &L : h->bwd_levels)
+        hipLaunchKernelGGL(k_bwd, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1);
+    if (prof) HIP_TRY(hipEventRecord(h->ev[4], s));
+    hipLaunchKernelGGL(k_leaf_final, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
+                       h->d_UT, h->d_y1, h->d_fwd, d_y);
+    HIP_TRY(hipGetLastError());
+    if (prof) {
+        HIP_TRY(hipEventRecord(h->ev[5], s));
+        HIP_TRY(hipEventSynchronize(h->ev[5]));
+        float t[5];
+        for (int i = 0; i < 5; i++) HIP_TRY(hipEventElapsedTime(&t[i], h->ev[i], h->ev[i + 1]));
+        h->info.t_fwd = t[0];
+        h->info.t_blocks = t[1];
+        h->info.t_reduce = t[2];
+        h->info.t_bwd = t[3];
+        h->info.t_final = t[4];
+        h->info.t_total = t[0] + t[1] + t[2] + t[3] + t[4];
+    }
+    return 0;
+}
+
+// column-major <-> row-major panel transposes for matmul staging
+__global__ void k_transpose(const double *__restrict__ in, long long ldi, double *__restrict__ out,
+                            long long ldo, int rows, int cols) {
+    // out[c * ldo + r] = in[r * ldi + c]
+    __shared__ double tile[32][33];
+    int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[k][threadIdx.x] = in[(long long)r * ldi + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[(long long)c * ldo + r] = tile[threadIdx.x][k];
+    }
+}
+
+extern "C" {
+
+const char *h2p_hip_last_error(void) { return g_err.c_str(); }
+
+int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mode, int prec,
+                   h2p_hip_handle **out) {
+    if (!out) return fail(H2P_HIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!g || !krnl) return fail(H2P_HIP_EINVAL, "NULL representation or kernel");
+    if (mode != H2P_HIP_AOT && mode != H2P_HIP_JIT) return fail(H2P_HIP_EINVAL, "bad mode");
+    if (prec != H2P_HIP_FP64) return fail(H2P_HIP_ENOSYS, "only FP64 is available in this build");
+    if (krnl->id < 0 || krnl->id > 3) return fail(H2P_HIP_EINVAL, "bad kernel id");
+    const int kd = g->krnl_dim;
+    if (g->pt_dim != 3) return fail(H2P_HIP_EINVAL, "pt_dim must be 3");
+    if ((krnl->id == KR) != (kd == 3) || (kd != 1 && kd != 3))
+        return fail(H2P_HIP_EINVAL, "krnl_dim does not match the kernel");
+    if (krnl->id == KR && g->xpt_dim < 4) return fail(H2P_HIP_EINVAL, "RPY needs xpt_dim 4");
+    if (g->n_point <= 0 || g->krnl_mat_size != kd * g->n_point || g->n_node <= 0)
+        return fail(H2P_HIP_EINVAL, "inconsistent sizes");
+    if ((long long)g->n_point * kd > 0x7fffffffLL) return fail(H2P_HIP_EINVAL, "too many points");
+
+    h2p_hip_handle *h = new h2p_hip_handle();
+    h->mode = mode;
+    h->prec = prec;
+    h->kid = krnl->id;
+    h->kd = kd;
+    h->xpt_dim = g->xpt_dim;
+    h->N = g->krnl_mat_size;
+    h->n_point = g->n_point;
+    if (krnl->id == KC) h->kp.p0 = krnl->n_param > 0 ? krnl->param[0] : 0.0;
+    else h->kp.p0 = krnl->param[0];
+    if (krnl->id == KR) h->kp.p1 = 1.0 / (6.0 * M_PI * krnl->param[0]);
+
+    auto bail = [&](int rc) { free_handle(h); return rc; };
+
+    const int nn = g->n_node, nlf = g->n_leaf_node, mc = g->max_child;
+    const int N = h->N;
+    // ---- tree helpers
+    std::vector<int> parent(nn, -1);
+    for (int i = 0; i < nn; i++)
+        for (int k = 0; k < g->n_child[i]; k++) {
+            int c = g->children[(size_t)i * mc + k];
+            if (c < 0 || c >= nn) return bail(fail(H2P_HIP_EINVAL, "bad child index"));
+            parent[c] = i;
+        }
+    int root = -1;
+    for (int i = 0; i < nn; i++)
+        if (parent[i] < 0) {
+            if (root >= 0) return bail(fail(H2P_HIP_EINVAL, "more than one root"));
+            root = i;
+        }
+    if (root < 0) return bail(fail(H2P_HIP_EINVAL, "no root"));
+    auto npt = [&](int n) { return g->pt_cluster[2 * n + 1] - g->pt_cluster[2 * n] + 1; };
+    for (int i = 0; i < nn; i++) {
+        if (g->U_ncol[i] < 0 || g->U_nrow[i] < 0) return bail(fail(H2P_HIP_EINVAL, "negative U size"));
+        if (g->U_ncol[i] > 0 && !g->U[i]) return bail(fail(H2P_HIP_EINVAL, "NULL U"));
+        if (g->U_ncol[i] > 0 && g->U_ncol[i] != g->J_npt[i] * kd)
+            return bail(fail(H2P_HIP_EINVAL, "U_ncol != kd * |J|"));
+        if (g->mat_cluster[2 * i] != kd * g->pt_cluster[2 * i]) return bail(fail(H2P_HIP_EINVAL, "mat_cluster mismatch"));
+    }
+    // ---- skeleton offsets: BFS so that children of a node are contiguous
+    std::vector<int> soff(nn, 0);
+    {
+        std::vector<int> q{root};
+        int cur = 0;
+        soff[root] = cur;
+        cur += g->U_ncol[root];
+        for (size_t qi = 0; qi < q.size(); qi++) {
+            int n = q[qi];
+            for (int k = 0; k < g->n_child[n]; k++) {
+                int c = g->children[(size_t)n * mc + k];
+                soff[c] = cur;
+                cur += g->U_ncol[c];
+                q.push_back(c);
+            }
+        }
+        if ((int)q.size() != nn) return bail(fail(H2P_HIP_EINVAL, "tree is not connected"));
+        h->S = cur;
+    }
+    const int S = h->S;
+    // ---- U and U^T blobs
+    std::vector<long long> uoff(nn, 0);
+    long long utot = 0;
+    for (int i = 0; i < nn; i++) {
+        uoff[i] = utot;
+        utot += (long long)g->U_nrow[i] * g->U_ncol[i];
+    }
+    std::vector<double> Ub(utot), UTb(utot);
+    for (int i = 0; i < nn; i++) {
+        const int nr = g->U_nrow[i], r = g->U_ncol[i];
+        if (r == 0) continue;
+        const double *src = g->U[i];
+        std::memcpy(Ub.data() + uoff[i], src, sizeof(double) * nr * (size_t)r);
+        double *dt = UTb.data() + uoff[i];
+        for (int a = 0; a < nr; a++)
+            for (int c = 0; c < r; c++) dt[(size_t)c * nr + a] = src[(size_t)a * r + c];
+    }
+    // ---- unified coordinates [points | skeleton points]
+    std::vector<int> joff(nn, 0);
+    long long jtot = 0;
+    for (int i = 0; i < nn; i++) {
+        joff[i] = (int)jtot;
+        jtot += (g->U_ncol[i] > 0) ? g->J_npt[i] : 0;
+    }
+    const long long CT = g->n_point + jtot;
+    h->CT = CT;
+    const int xd = g->xpt_dim;
+    std::vector<double> cu((size_t)xd * CT);
+    for (int d = 0; d < xd; d++) {
+        std::memcpy(cu.data() + (size_t)d * CT, g->coord + (size_t)d * g->n_point, sizeof(double) * g->n_point);
+        for (int i = 0; i < nn; i++) {
+            if (g->U_ncol[i] == 0) continue;
+            const int m = g->J_npt[i];
+            std::memcpy(cu.data() + (size_t)d * CT + g->n_point + joff[i], g->J_coord[i] + (size_t)d * m, sizeof(double) * m);
+        }
+    }
+    // ---- upward sweep tasks per level
+    std::vector<FwdTask> fwdt;
+    for (int lv = g->max_level; lv >= g->min_adm_level; lv--) {
+        Level L{(int)fwdt.size(), 0};
+        for (int j = 0; j < g->level_n_node[lv]; j++) {
+            int n = g->level_nodes[(size_t)lv * nlf + j];
+            if (g->U_ncol[n] == 0) continue;
+            FwdTask t{};
+            t.u_off = uoff[n];
+            t.nrow = g->U_nrow[n];
+            t.r = g->U_ncol[n];
+            t.out_off = N + soff[n];
+            if (g->n_child[n] == 0) {
+                if (t.nrow != npt(n) * kd) return bail(fail(H2P_HIP_EINVAL, "leaf U_nrow != kd * points"));
+                t.in_off = g->mat_cluster[2 * n];
+            } else {
+                int sum = 0;
+                for (int k = 0; k < g->n_child[n]; k++) sum += g->U_ncol[g->children[(size_t)n * mc + k]];
+                if (sum != t.nrow) return bail(fail(H2P_HIP_EINVAL, "non-leaf U_nrow != sum of child ranks"));
+                t.in_off = N + soff[g->children[(size_t)n * mc]];
+            }
+            fwdt.push_back(t);
+        }
+        L.n = (int)fwdt.size() - L.off;
+        if (L.n > 0) h->fwd_levels.push_back(L);
+    }
+    // ---- grouped block tasks (B then D) and partial destinations
+    std::vector<BlkTask> blk;
+    std::vector<std::vector<long long>> dest_y1(nn), dest_leaf(nn);
+    long long arena = 0;
+    size_t btot = (g->n_r_adm_pair > 0) ? g->B_ptr[g->n_r_adm_pair] : 0;
+    const int nD = nlf + g->n_r_inadm_pair;
+    size_t dtot = g->D_ptr[nD];
+    double B_pairs = 0, D_pairs = 0;
+    for (int i = 0; i < g->n_r_adm_pair; i++) {
+        const int n0 = g->r_adm_pairs[2 * i], n1 = g->r_adm_pairs[2 * i + 1];
+        const int l0 = g->node_level[n0], l1 = g->node_level[n1];
+        BlkTask t{};
+        t.nrow = g->B_nrow[i];
+        t.ncol = g->B_ncol[i];
+        if (l0 < l1) { t.c0 = g->pt_cluster[2 * n0]; t.n0 = npt(n0); t.in1 = g->mat_cluster[2 * n0]; }
+        else { t.c0 = g->n_point + joff[n0]; t.n0 = g->J_npt[n0]; t.in1 = N + soff[n0]; }
+        if (l0 > l1) { t.c1 = g->pt_cluster[2 * n1]; t.n1 = npt(n1); t.in0 = g->mat_cluster[2 * n1]; }
+        else { t.c1 = g->n_point + joff[n1]; t.n1 = g->J_npt[n1]; t.in0 = N + soff[n1]; }
+        if (t.nrow != t.n0 * kd || t.ncol != t.n1 * kd)
+            return bail(fail(H2P_HIP_EINVAL, "B block size does not match its point sets"));
+        if ((l0 >= l1 && g->U_ncol[n0] == 0) || (l0 <= l1 && g->U_ncol[n1] == 0)) {
+            if (t.nrow * (long long)t.ncol != 0) return bail(fail(H2P_HIP_EINVAL, "B block on a node without U"));
+        }
+        if (t.nrow == 0 || t.ncol == 0) continue;
+        t.data_off = (long long)g->B_ptr[i];
+        t.flags = 1;
+        t.out0 = arena; arena += t.nrow;
+        t.out1 = arena; arena += t.ncol;
+        (l0 < l1 ? dest_leaf[n0] : dest_y1[n0]).push_back(t.out0);
+        (l0 > l1 ? dest_leaf[n1] : dest_y1[n1]).push_back(t.out1);
+        B_pairs += (double)t.n0 * t.n1;
+        blk.push_back(t);
+    }
+    for (int i = 0; i < nD; i++) {
+        BlkTask t{};
+        int n0, n1;
+        if (i < nlf) n0 = n1 = g->leaf_nodes[i];
+        else { n0 = g->r_inadm_pairs[2 * (i - nlf)]; n1 = g->r_inadm_pairs[2 * (i - nlf) + 1]; }
+        if (g->n_child[n0] != 0 || g->n_child[n1] != 0) return bail(fail(H2P_HIP_EINVAL, "D block on a non-leaf"));
+        t.nrow = g->D_nrow[i];
+        t.ncol = g->D_ncol[i];
+        t.c0 = g->pt_cluster[2 * n0]; t.n0 = npt(n0);
+        t.c1 = g->pt_cluster[2 * n1]; t.n1 = npt(n1);
+        if (t.nrow != t.n0 * kd || t.ncol != t.n1 * kd) return bail(fail(H2P_HIP_EINVAL, "D block size mismatch"));
+        t.in0 = g->mat_cluster[2 * n1];
+        t.in1 = g->mat_cluster[2 * n0];
+        t.data_off = (long long)(btot + g->D_ptr[i]);
+        t.out0 = arena; arena += t.nrow;
+        dest_leaf[n0].push_back(t.out0);
+        if (i >= nlf) {
+            t.flags = 1;
+            t.out1 = arena; arena += t.ncol;
+            dest_leaf[n1].push_back(t.out1);
+        } else {
+            t.flags = 0;
+            t.out1 = -1;
+        }
+        D_pairs += (double)t.n0 * t.n1;
+        if (t.nrow == 0 || t.ncol == 0) continue;
+        blk.push_back(t);
+    }
+    h->arena_len = arena;
+    // longest-processing-time order for the grouped launch
+    std::stable_sort(blk.begin(), blk.end(), [](const BlkTask &a, const BlkTask &b) {
+        return (long long)a.nrow * a.ncol > (long long)b.nrow * b.ncol;
+    });
+    h->n_blk = (int)blk.size();
+    // ---- reduction into y1, and the contribution list
+    std::vector<long long> contrib;
+    std::vector<RedTask> red;
+    for (int n = 0; n < nn; n++) {
+        if (g->U_ncol[n] == 0) continue;
+        RedTask t{};
+        t.dst = soff[n];
+        t.len = g->U_ncol[n];
+        t.cbeg = (int)contrib.size();
+        for (long long o : dest_y1[n]) contrib.push_back(o);
+        t.cend = (int)contrib.size();
+        red.push_back(t);
+    }
+    h->n_red = (int)red.size();
+    // ---- active nodes of the downward sweep (y1 "set", :251-256, :802-838)
+    std::vector<char> active(nn, 0);
+    for (int lv = g->min_adm_level; lv <= g->max_level; lv++)
+        for (int j = 0; j < g->level_n_node[lv]; j++) {
+            int n = g->level_nodes[(size_t)lv * nlf + j];
+            bool a = g->node_n_r_adm[n] > 0;
+            if (parent[n] >= 0 && g->node_level[parent[n]] >= g->min_adm_level && active[parent[n]]) a = true;
+            active[n] = (a && g->U_ncol[n] > 0) ? 1 : 0;
+        }
+    std::vector<BwdTask> bwdt;
+    for (int lv = g->min_adm_level; lv <= g->max_level; lv++) {
+        Level L{(int)bwdt.size(), 0};
+        for (int j = 0; j < g->level_n_node[lv]; j++) {
+            int n = g->level_nodes[(size_t)lv * nlf + j];
+            if (!active[n] || g->n_child[n] == 0) continue;
+            BwdTask t{};
+            t.ut_off = uoff[n];
+            t.nrow = g->U_nrow[n];
+            t.r = g->U_ncol[n];
+            t.in_off = soff[n];
+            t.out_off = soff[g->children[(size_t)n * mc]];
+            bwdt.push_back(t);
+        }
+        L.n = (int)bwdt.size() - L.off;
+        if (L.n > 0) h->bwd_levels.push_back(L);
+    }
+    std::vector<LeafTask> leaf;
+    for (int i = 0; i < nlf; i++) {
+        int n = g->leaf_nodes[i];
+        LeafTask t{};
+        t.nrow = npt(n) * kd;
+        t.r = active[n] ? g->U_ncol[n] : 0;
+        t.active = active[n];
+        t.ut_off = uoff[n];
+        t.y1_off = soff[n];
+        t.vec_s = g->mat_cluster[2 * n];
+        t.cbeg = (int)contrib.size();
+        for (long long o : dest_leaf[n]) contrib.push_back(o);
+        t.cend = (int)contrib.size();
+        leaf.push_back(t);
+    }
+    h->n_leaf = (int)leaf.size();
+    // ---- every access of every task stays inside its buffer
+    {
+        const long long XIN = (long long)N + S, Y1 = S, AR = arena, NC = (long long)contrib.size();
+        const long long BLOB = (long long)(btot + dtot);
+        if (XIN > 0x7fffffffLL) return bail(fail(H2P_HIP_EINVAL, "vector space exceeds 2^31 entries"));
+        auto in = [](long long off, long long len, long long cap) { return off >= 0 && len >= 0 && off + len <= cap; };
+        for (const FwdTask &t : fwdt)
+            if (!in(t.in_off, t.nrow, XIN) || !in(t.out_off, t.r, XIN) || !in(t.u_off, (long long)t.nrow * t.r, utot))
+                return bail(fail(H2P_HIP_EINVAL, "upward task out of range"));
+        for (const BwdTask &t : bwdt)
+            if (!in(t.in_off, t.r, Y1) || !in(t.out_off, t.nrow, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot))
+                return bail(fail(H2P_HIP_EINVAL, "downward task out of range"));
+        for (const RedTask &t : red) {
+            if (!in(t.dst, t.len, Y1) || t.cbeg < 0 || t.cend < t.cbeg || t.cend > NC)
+                return bail(fail(H2P_HIP_EINVAL, "reduction task out of range"));
+            for (int k = t.cbeg; k < t.cend; k++)
+                if (!in(contrib[k], t.len, AR)) return bail(fail(H2P_HIP_EINVAL, "reduction source out of range"));
+        }
+        for (const LeafTask &t : leaf) {
+            if (!in(t.vec_s, t.nrow, N) || t.cbeg < 0 || t.cend < t.cbeg || t.cend > NC)
+                return bail(fail(H2P_HIP_EINVAL, "leaf task out of range"));
+            if (t.active && (!in(t.y1_off, t.r, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot)))
+                return bail(fail(H2P_HIP_EINVAL, "leaf task out of range"));
+            for (int k = t.cbeg; k < t.cend; k++)
+                if (!in(contrib[k], t.nrow, AR)) return bail(fail(H2P_HIP_EINVAL, "leaf source out of range"));
+        }
+        for (const BlkTask &t : blk) {
+            bool ok = in(t.in0, t.ncol, XIN) && in(t.out0, t.nrow, AR) && in(t.c0, t.n0, CT) && in(t.c1, t.n1, CT) &&
+                      t.nrow == t.n0 * kd && t.ncol == t.n1 * kd;
+            if (t.flags & 1) ok = ok && in(t.in1, t.nrow, XIN) && in(t.out1, t.ncol, AR);
+            if (mode == H2P_HIP_AOT) ok = ok && in(t.data_off, (long long)t.nrow * t.ncol, BLOB);
+            if (!ok) return bail(fail(H2P_HIP_EINVAL, "block task out of range"));
+        }
+        for (int i = 0; i < N; i++)
+            if (fwdidx_ok(g->fwd_pmt_idx[i], N) == false) return bail(fail(H2P_HIP_EINVAL, "fwd_pmt_idx out of range"));
+    }
+    // ---- device memory
+    size_t *acct = &h->device_bytes;
+    std::vector<int> fwdidx(g->fwd_pmt_idx, g->fwd_pmt_idx + N);
+    int rc;
+    if ((rc = dev_upload(&h->d_coord, cu, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_U, Ub, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_UT, UTb, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_fwd, fwdidx, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_blk, blk, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_fwdt, fwdt, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_bwdt, bwdt, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_red, red, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_leaf, leaf, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_contrib, contrib, acct))) return bail(rc);
+    if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
+    if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
+    if ((rc = dev_alloc(&h->d_arena, (size_t)arena, acct))) return bail(rc);
+    if (hipMemset(h->d_xin, 0, sizeof(double) * ((size_t)N + S)) != hipSuccess)
+        return bail(fail(H2P_HIP_EDEVICE, "hipMemset failed"));
+    if (mode == H2P_HIP_AOT) {
+        if ((rc = dev_alloc(&h->d_blob, btot + dtot, acct))) return bail(rc);
+        if (g->B_data && g->D_data) {
+            if (btot && hipMemcpy(h->d_blob, g->B_data, sizeof(double) * btot, hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(H2P_HIP_EDEVICE, "B upload failed"));
+            if (dtot && hipMemcpy(h->d_blob + btot, g->D_data, sizeof(double) * dtot, hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(H2P_HIP_EDEVICE, "D upload failed"));
+        } else if (!g->B_data && !g->D_data) {
+            if (h->n_blk > 0) dispatch_build(h, 0);
+            if (hipDeviceSynchronize() != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "device block build failed"));
+        } else {
+            return bail(fail(H2P_HIP_EINVAL, "give both B_data and D_data or neither"));
+        }
+    }
+    if (hipEventCreate(&h->ev[0]) == hipSuccess) {
+        h->ev_ok = true;
+        for (int i = 1; i < 7; i++)
+            if (hipEventCreate(&h->ev[i]) != hipSuccess) h->ev_ok = false;
+    }
+    if (hipStreamCreate(&h->own_stream) != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "stream create failed"));
+    // ---- statistics (H2Pack's mat_size / JIT_flops model)
+    h2p_hip_info &I = h->info;
+    I.U_elems = (size_t)utot;
+    I.B_elems = btot;
+    I.D_elems = dtot;
+    // mat_size[MV_FWD/MID/BWD/DEN] of src/H2Pack_build.c:251-255, 755-758, 965-1002
+    size_t vec = 0;
+    for (int i = 0; i < nn; i++) vec += 2 * (size_t)(g->U_nrow[i] + g->U_ncol[i]);
+    for (int i = 0; i < g->n_r_adm_pair; i++) vec += 2 * (size_t)(g->B_nrow[i] + g->B_ncol[i]);
+    for (int i = 0; i < nD; i++) vec += (size_t)(i < nlf ? 1 : 2) * (g->D_nrow[i] + g->D_ncol[i]);
+    I.hbm_bytes_model = 8 * (2 * (size_t)utot + (mode == H2P_HIP_AOT ? btot + dtot : 0) + vec);
+    I.B_pairs = B_pairs;
+    I.D_pairs = D_pairs;
+    static const int flops[4] = {14, 14, 17, 82};
+    I.jit_flops_model = flops[h->kid] * (B_pairs + D_pairs);
+    I.device_bytes = h->device_bytes;
+    I.n_block_tasks = h->n_blk;
+    I.n_levels = g->max_level + 1;
+    I.mode = mode;
+    I.prec = prec;
+    *out = h;
+    return 0;
+}
+
+int h2p_hip_matvec_dev(h2p_hip_handle *h, const double *d_x, double *d_y, void *stream) {
+    if (!h || !d_x || !d_y) return fail(H2P_HIP_EINVAL, "NULL argument");
+    return run_matvec(h, d_x, d_y, (hipStream_t)stream);
+}
+
+int h2p_hip_matvec(h2p_hip_handle *h, const double *x, double *y) {
+    if (!h || !x || !y) return fail(H2P_HIP_EINVAL, "NULL argument");
+    int rc;
+    if (h->mm_cap < 1) {
+        if (h->d_mm_x) { (void)hipFree(h->d_mm_x); h->d_mm_x = nullptr; }
+        if (h->d_mm_y) { (void)hipFree(h->d_mm_y); h->d_mm_y = nullptr; }
+        size_t dummy = 0;
+        if ((rc = dev_alloc(&h->d_mm_x, (size_t)h->N, &dummy))) return rc;
+        if ((rc = dev_alloc(&h->d_mm_y, (size_t)h->N, &dummy))) return rc;
+        h->mm_cap = 1;
+    }
+    hipStream_t s = h->own_stream;
+    HIP_TRY(hipMemcpyAsync(h->d_mm_x, x, sizeof(double) * h->N, hipMemcpyHostToDevice, s));
+    if ((rc = run_matvec(h, h->d_mm_x, h->d_mm_y, s))) return rc;
+    HIP_TRY(hipMemcpyAsync(y, h->d_mm_y, sizeof(double) * h->N, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d_X, int ldx,
+                       double *d_Y, int ldy, void *stream) {
+    if (!h || (n_vec > 0 && (!d_X || !d_Y))) return fail(H2P_HIP_EINVAL, "NULL argument");
+    if (n_vec < 0) return fail(H2P_HIP_EINVAL, "n_vec < 0");
+    if (n_vec == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    const int N = h->N;
+    if (layout == H2P_HIP_COL_MAJOR) {
+        if (ldx < N || ldy < N) return fail(H2P_HIP_EINVAL, "ld < krnl_mat_size");
+        for (int v = 0; v < n_vec; v++) {
+            int rc = run_matvec(h, d_X + (size_t)v * ldx, d_Y + (size_t)v * ldy, s);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    if (layout != H2P_HIP_ROW_MAJOR) return fail(H2P_HIP_EINVAL, "bad layout");
+    if (ldx < n_vec || ldy < n_vec) return fail(H2P_HIP_EINVAL, "ld < n_vec");
+    // stage row-major panels as columns
+    if (h->mm_cap < (size_t)n_vec) {
+        if (h->d_mm_x) { (void)hipFree(h->d_mm_x); h->d_mm_x = nullptr; }
+        if (h->d_mm_y) { (void)hipFree(h->d_mm_y); h->d_mm_y = nullptr; }
+        size_t dummy = 0;
+        int rc;
+        if ((rc = dev_alloc(&h->d_mm_x, (size_t)N * n_vec, &dummy))) return rc;
+        if ((rc = dev_alloc(&h->d_mm_y, (size_t)N * n_vec, &dummy))) return rc;
+        h->mm_cap = n_vec;
+    }
+    dim3 g1((n_vec + 31) / 32, (N + 31) / 32), b(32, 8);
+    hipLaunchKernelGGL(k_transpose, g1, b, 0, s, d_X, (long long)ldx, h->d_mm_x, (long long)N, N, n_vec);
+    for (int v = 0; v < n_vec; v++) {
+        int rc = run_matvec(h, h->d_mm_x + (size_t)v * N, h->d_mm_y + (size_t)v * N, s);
+        if (rc) return rc;
+    }
+    dim3 g2((N + 31) / 32, (n_vec + 31) / 32);
+    hipLaunchKernelGGL(k_transpose, g2, b, 0, s, h->d_mm_y, (long long)N, d_Y, (long long)ldy, n_vec, N);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, int ldx, double *Y,
+                   int ldy) {
+    if (!h || (n_vec > 0 && (!X || !Y))) return fail(H2P_HIP_EINVAL, "NULL argument");
+    if (n_vec <= 0) return n_vec < 0 ? fail(H2P_HIP_EINVAL, "n_vec < 0") : 0;
+    const int N = h->N;
+    const bool rm = layout == H2P_HIP_ROW_MAJOR;
+    if (!rm && layout != H2P_HIP_COL_MAJOR) return fail(H2P_HIP_EINVAL, "bad layout");
+    if (rm ? (ldx < n_vec || ldy < n_vec) : (ldx < N || ldy < N)) return fail(H2P_HIP_EINVAL, "bad leading dimension");
+    // pack the caller's panel densely, run on the device, unpack
+    double *dX = nullptr, *dY = nullptr;
+    const size_t elems = (size_t)N * n_vec;
+    if (hipMalloc(&dX, sizeof(double) * elems) != hipSuccess) return fail(H2P_HIP_ENOMEM, "hipMalloc failed");
+    if (hipMalloc(&dY, sizeof(double) * elems) != hipSuccess) { (void)hipFree(dX); return fail(H2P_HIP_ENOMEM, "hipMalloc failed"); }
+    hipStream_t s = h->own_stream;
+    int rc = 0;
+    const int rows = rm ? N : n_vec, cols = rm ? n_vec : N;
+    if (hipMemcpy2DAsync(dX, sizeof(double) * cols, X, sizeof(double) * ldx, sizeof(double) * cols, rows,
+                         hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = fail(H2P_HIP_EDEVICE, "upload of X failed");
+    if (!rc) rc = h2p_hip_matmul_dev(h, layout, n_vec, dX, cols, dY, cols, s);
+    if (!rc && hipMemcpy2DAsync(Y, sizeof(double) * ldy, dY, sizeof(double) * cols, sizeof(double) * cols, rows,
+                                hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = fail(H2P_HIP_EDEVICE, "download of Y failed");
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(H2P_HIP_EDEVICE, "stream sync failed");
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    return rc;
+}
+
+int h2p_hip_set_profiling(h2p_hip_handle *h, int on) {
+    if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
+    h->profiling = on ? 1 : 0;
+    return 0;
+}
+
+int h2p_hip_get_info(const h2p_hip_handle *h, h2p_hip_info *info) {
+    if (!h || !info) return fail(H2P_HIP_EINVAL, "NULL argument");
+    *info = h->info;
+    return 0;
+}
+
+void h2p_hip_free(h2p_hip_handle *h) { free_handle(h); }
+
+int h2p_hip_direct_nbody(const h2p_kernel_desc *krnl, int xpt_dim, const double *src, int src_ld,
+                         int n_src, const double *x, const double *dst, int dst_ld, int n_dst,
+                         double *y) {
+    if (!krnl || !src || !x || !dst || !y) return fail(H2P_HIP_EINVAL, "NULL argument");
+    if (n_src <= 0 || n_dst <= 0 || src_ld < n_src || dst_ld < n_dst) return fail(H2P_HIP_EINVAL, "bad sizes");
+    const int kid = krnl->id;
+    if (kid < 0 || kid > 3) return fail(H2P_HIP_EINVAL, "bad kernel id");
+    const int kd = kid == KR ? 3 : 1;
+    if (kid == KR && xpt_dim < 4) return fail(H2P_HIP_EINVAL, "RPY needs xpt_dim 4");
+    KParam kp{};
+    kp.p0 = (kid == KC) ? (krnl->n_param > 0 ? krnl->param[0] : 0.0) : krnl->param[0];
+    if (kid == KR) kp.p1 = 1.0 / (6.0 * M_PI * krnl->param[0]);
+    const int xd = kid == KR ? 4 : 3;
+    const int chunk = 8192;
+    const int nchunk = (n_src + chunk - 1) / chunk;
+    double *ds = nullptr, *dx = nullptr, *dd = nullptr, *dp = nullptr, *dy = nullptr;
+    int rc = 0;
+    size_t acct = 0;
+    std::vector<double> sc((size_t)xd * n_src), dc((size_t)xd * n_dst);
+    for (int d = 0; d < xd; d++) {
+        std::memcpy(sc.data() + (size_t)d * n_src, src + (size_t)d * src_ld, sizeof(double) * n_src);
+        std::memcpy(dc.data() + (size_t)d * n_dst, dst + (size_t)d * dst_ld, sizeof(double) * n_dst);
+    }
+    std::vector<double> xv(x, x + (size_t)n_src * kd);
+    if ((rc = dev_upload(&ds, sc, &acct)) || (rc = dev_upload(&dd, dc, &acct)) || (rc = dev_upload(&dx, xv, &acct)) ||
+        (rc = dev_alloc(&dp, (size_t)nchunk * n_dst * kd, &acct)) || (rc = dev_alloc(&dy, (size_t)n_dst * kd, &acct))) {
+    } else {
+        dim3 grid((n_dst + 255) / 256, nchunk);
+        switch (kid) {
+            case KC: hipLaunchKernelGGL((k_nbody_partial<KC, 1>), grid, dim3(256), 0, 0, ds, (long long)n_src, n_src, dx, dd, (long long)n_dst, n_dst, kp, chunk, dp); break;
+            case KG: hipLaunchKernelGGL((k_nbody_partial<KG, 1>), grid, dim3(256), 0, 0, ds, (long long)n_src, n_src, dx, dd, (long long)n_dst, n_dst, kp, chunk, dp); break;
+            case KM: hipLaunchKernelGGL((k_nbody_partial<KM, 1>), grid, dim3(256), 0, 0, ds, (long long)n_src, n_src, dx, dd, (long long)n_dst, n_dst, kp, chunk, dp); break;
+            default: hipLaunchKernelGGL((k_nbody_partial<KR, 3>), grid, dim3(256), 0, 0, ds, (long long)n_src, n_src, dx, dd, (long long)n_dst, n_dst, kp, chunk, dp); break;
+        }
+        int len = n_dst * kd;
+        hipLaunchKernelGGL(k_nbody_reduce, dim3((len + 255) / 256), dim3(256), 0, 0, dp, nchunk, len, dy);
+        if (hipDeviceSynchronize() != hipSuccess) rc = fail(H2P_HIP_EDEVICE, "n-body kernel failed");
+        else if (hipMemcpy(y, dy, sizeof(double) * len, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(H2P_HIP_EDEVICE, "n-body download failed");
+    }
+    void *ps[] = {ds, dx, dd, dp, dy};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,420 @@
+// Global kernels of the MI355X H2 engine (matvec path).
+//
+// Vector spaces on the device:
+//   xin   = [ px (N, permuted input) | y0 (S, skeleton coefficients) ]
+//   y1    = S skeleton accumulators (downward sweep)
+//   arena = partial outputs of the grouped B/D block launch
+// Skeleton offsets are assigned so that the children of every node occupy a
+// contiguous range in child order; the stacked operand of a non-leaf U is
+// therefore one contiguous slice of y0 / y1.
+#pragma once
+#include "h2p_device.h"
+
+namespace h2p {
+
+// One B or D block of the grouped launch.
+struct BlkTask {
+    long long data_off;   // AOT: element offset of the row-major block
+    long long out0, out1; // arena offsets of the two partial outputs (out1 < 0: none)
+    int nrow, ncol;       // block dimensions in vector entries (points * kd)
+    int c0, c1;           // JIT: first point of each set in the unified coordinates
+    int n0, n1;           // JIT: point counts
+    int in0, in1;         // xin offsets of the operands (in0: ncol long, in1: nrow long)
+    int flags;            // bit 0: has out1
+    int pad;
+};
+
+// Upward-sweep task: y0[out] = U^T in, U row-major nrow x r.
+struct FwdTask {
+    long long u_off;
+    int nrow, r, in_off, out_off;
+};
+
+// Downward-sweep task: y1[out .. out+nrow) += U y1[in .. in+r), U^T stored row-major.
+struct BwdTask {
+    long long ut_off;
+    int nrow, r, in_off, out_off;
+};
+
+// Segmented reduction into y1: y1[dst + e] = sum_k arena[contrib[k] + e].
+struct RedTask {
+    int dst, len, cbeg, cend;
+};
+
+// Leaf output: y[fwd[vec_s + i]] = (U y1)[i] + sum_k arena[contrib[k] + i].
+struct LeafTask {
+    long long ut_off;
+    int nrow, r, y1_off, vec_s, cbeg, cend, active, pad;
+};
+
+// ---------------------------------------------------------------------------
+__global__ void k_gather(const double *__restrict__ x, const int *__restrict__ idx,
+                         double *__restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x[idx[i]];
+}
+
+// Upward sweep for one level, src/H2Pack_matvec.c:95-177 (one workgroup per
+// node; lanes own U columns, the four waves split the rows).
+__global__ void __launch_bounds__(256) k_fwd(const FwdTask *__restrict__ tasks,
+                                             const double *__restrict__ U,
+                                             double *__restrict__ xin) {
+    __shared__ double red[4][H2P_WAVE];
+    const FwdTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double *Ut = U + t.u_off;
+    const double *in = xin + t.in_off;
+    for (int cs = 0; cs < t.r; cs += H2P_WAVE) {
+        const int c = cs + lane;
+        double acc = 0.0;
+        if (c < t.r) {
+            int i = w;
+            for (; i + 12 < t.nrow; i += 16) {
+                double u0 = Ut[(long long)i * t.r + c];
+                double u1 = Ut[(long long)(i + 4) * t.r + c];
+                double u2 = Ut[(long long)(i + 8) * t.r + c];
+                double u3 = Ut[(long long)(i + 12) * t.r + c];
+                acc += u0 * in[i];
+                acc += u1 * in[i + 4];
+                acc += u2 * in[i + 8];
+                acc += u3 * in[i + 12];
+            }
+            for (; i < t.nrow; i += 4) acc += Ut[(long long)i * t.r + c] * in[i];
+        }
+        red[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && c < t.r)
+            xin[t.out_off + c] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        __syncthreads();
+    }
+}
+
+// Downward sweep for one level, src/H2Pack_matvec.c:767-861 (push-down into
+// the children's contiguous y1 slice; one writer per child).
+__global__ void __launch_bounds__(256) k_bwd(const BwdTask *__restrict__ tasks,
+                                             const double *__restrict__ UT,
+                                             double *__restrict__ y1) {
+    const BwdTask t = tasks[blockIdx.x];
+    const double *ut = UT + t.ut_off;
+    const double *in = y1 + t.in_off;
+    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) {
+        double acc = 0.0;
+        for (int c = 0; c < t.r; c++) acc += ut[(long long)c * t.nrow + i] * in[c];
+        y1[t.out_off + i] += acc;
+    }
+}
+
+// Fixed-order segmented reduction of the block partials into y1.
+__global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ tasks,
+                                                   const long long *__restrict__ contrib,
+                                                   const double *__restrict__ arena,
+                                                   double *__restrict__ y1) {
+    const RedTask t = tasks[blockIdx.x];
+    for (int e = threadIdx.x; e < t.len; e += blockDim.x) {
+        double s = 0.0;
+        for (int k = t.cbeg; k < t.cend; k++) s += arena[contrib[k] + e];
+        y1[t.dst + e] = s;
+    }
+}
+
+// Leaf outputs: leaf part of the downward sweep plus every block partial that
+// lands on the leaf, written straight to the caller's order (y[fwd[i]] = py[i]
+// is the backward permutation of src/H2Pack_matvec.c:1156-1159).
+__global__ void __launch_bounds__(256) k_leaf_final(const LeafTask *__restrict__ tasks,
+                                                    const long long *__restrict__ contrib,
+                                                    const double *__restrict__ arena,
+                                                    const double *__restrict__ UT,
+                                                    const double *__restrict__ y1,
+                                                    const int *__restrict__ fwd,
+                                                    double *__restrict__ y) {
+    const LeafTask t = tasks[blockIdx.x];
+    const double *ut = UT + t.ut_off;
+    const double *yv = y1 + t.y1_off;
+    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) {
+        double v = 0.0;
+        if (t.active)
+            for (int c = 0; c < t.r; c++) v += ut[(long long)c * t.nrow + i] * yv[c];
+        for (int k = t.cbeg; k < t.cend; k++) v += arena[contrib[k] + i];
+        y[fwd[t.vec_s + i]] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped bi-GEMV over stored blocks (AOT), the products of
+// CBLAS_BI_GEMV (src/H2Pack_matvec.c:26-65) as used by the intermediate
+// (:303-381) and dense (:864-934) phases.  One workgroup per block: lanes
+// own columns (NS strips of 64), the four waves take every fourth row; the
+// row sums are wave reductions, the column sums are reduced across waves in
+// LDS in a fixed order.
+template <int NS>
+__device__ __forceinline__ void blk_aot_pass(const BlkTask &t, int cbase, int ncols,
+                                             const double *__restrict__ A,
+                                             const double *__restrict__ xin,
+                                             double *__restrict__ arena,
+                                             double (*red)[4][H2P_WAVE], bool first) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double x0[NS], acc1[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        int c = k * H2P_WAVE + lane;
+        x0[k] = (c < ncols) ? xin[t.in0 + cbase + c] : 0.0;
+        acc1[k] = 0.0;
+    }
+    const double *in1 = xin + t.in1;
+    const bool has1 = t.flags & 1;
+    for (int i = w; i < t.nrow; i += 4) {
+        const double *row = A + (long long)i * t.ncol + cbase;
+        const double v1 = has1 ? in1[i] : 0.0;
+        double p = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            int c = k * H2P_WAVE + lane;
+            double a = (c < ncols) ? row[c] : 0.0;
+            p += a * x0[k];
+            acc1[k] += a * v1;
+        }
+        p = wave_sum(p);
+        if (lane == 0) {
+            if (first) arena[t.out0 + i] = p;
+            else arena[t.out0 + i] += p;
+        }
+    }
+    if (has1) {
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            red[k][w][lane] = acc1[k];
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+            int k = c >> 6, l = c & 63;
+            arena[t.out1 + cbase + c] = (red[k][0][l] + red[k][1][l]) + (red[k][2][l] + red[k][3][l]);
+        }
+        __syncthreads();
+    }
+}
+
+#define H2P_AOT_MAXNS 8
+__global__ void __launch_bounds__(256) k_blocks_aot(const BlkTask *__restrict__ tasks,
+                                                    const double *__restrict__ blob,
+                                                    const double *__restrict__ xin,
+                                                    double *__restrict__ arena) {
+    __shared__ double red[H2P_AOT_MAXNS][4][H2P_WAVE];
+    const BlkTask t = tasks[blockIdx.x];
+    const double *A = blob + t.data_off;
+    for (int cb = 0; cb < t.ncol; cb += H2P_AOT_MAXNS * H2P_WAVE) {
+        const int nc = min(t.ncol - cb, H2P_AOT_MAXNS * H2P_WAVE);
+        const int ns = (nc + 63) >> 6;
+        const bool first = (cb == 0);
+        if (ns <= 1) blk_aot_pass<1>(t, cb, nc, A, xin, arena, red, first);
+        else if (ns <= 2) blk_aot_pass<2>(t, cb, nc, A, xin, arena, red, first);
+        else if (ns <= 4) blk_aot_pass<4>(t, cb, nc, A, xin, arena, red, first);
+        else blk_aot_pass<8>(t, cb, nc, A, xin, arena, red, first);
+        // the next column pass adds into out0 rows written by the same lanes
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped kernel-evaluating bi-matvec (JIT), the krnl_bimv / krnl_eval
+// products of src/H2Pack_matvec.c:586-764 (B) and :1002-1145 (D).  Lanes own
+// source points (NS strips of 64), the four waves take every fourth target
+// point.  KD = 1 for scalar kernels, 3 for RPY.
+template <int KID, int KD, int NS>
+__device__ __forceinline__ void blk_jit_pass(const BlkTask &t, int pbase, int npts,
+                                             const double *__restrict__ cx,
+                                             long long cstride,
+                                             const KParam kp,
+                                             const double *__restrict__ xin,
+                                             double *__restrict__ arena,
+                                             double (*red)[KD][4][H2P_WAVE], bool first) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+    double sx[NS], sy[NS], sz[NS], sa[NS];
+    double x0[NS][KD], acc1[NS][KD];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        int p = k * H2P_WAVE + lane;
+        bool ok = p < npts;
+        int g = t.c1 + pbase + (ok ? p : 0);
+        sx[k] = cx[g]; sy[k] = cy[g]; sz[k] = cz[g];
+        if constexpr (KID == KR) sa[k] = ca[g];
+#pragma unroll
+        for (int b = 0; b < KD; b++) {
+            x0[k][b] = ok ? xin[t.in0 + (pbase + p) * KD + b] : 0.0;
+            acc1[k][b] = 0.0;
+        }
+    }
+    const bool has1 = t.flags & 1;
+    for (int i = w; i < t.n0; i += 4) {
+        const int gi = t.c0 + i;
+        const double tx = cx[gi], ty = cy[gi], tz = cz[gi];
+        double v1[KD];
+#pragma unroll
+        for (int a = 0; a < KD; a++) v1[a] = has1 ? xin[t.in1 + i * KD + a] : 0.0;
+        double p[KD];
+#pragma unroll
+        for (int a = 0; a < KD; a++) p[a] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const bool ok = (k * H2P_WAVE + lane) < npts;
+            double dx = tx - sx[k], dy = ty - sy[k], dz = tz - sz[k];
+            if constexpr (KD == 1) {
+                double kv = kval<KID>(dx * dx + dy * dy + dz * dz, kp);
+                kv = ok ? kv : 0.0;
+                p[0] += kv * x0[k][0];
+                acc1[k][0] += kv * v1[0];
+            } else {
+                const double ta = ca[gi];
+                double t1, t2, ux, uy, uz;
+                rpy_coef(dx, dy, dz, ta, sa[k], kp.p1, t1, t2, ux, uy, uz);
+                t1 = ok ? t1 : 0.0;
+                t2 = ok ? t2 : 0.0;
+                // K = t1 I + t2 u u^T (symmetric)
+                double ud0 = ux * x0[k][0] + uy * x0[k][1] + uz * x0[k][2];
+                double ud1 = ux * v1[0] + uy * v1[1] + uz * v1[2];
+                p[0] += t1 * x0[k][0] + t2 * ux * ud0;
+                p[1] += t1 * x0[k][1] + t2 * uy * ud0;
+                p[2] += t1 * x0[k][2] + t2 * uz * ud0;
+                acc1[k][0] += t1 * v1[0] + t2 * ux * ud1;
+                acc1[k][1] += t1 * v1[1] + t2 * uy * ud1;
+                acc1[k][2] += t1 * v1[2] + t2 * uz * ud1;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < KD; a++) {
+            double s = wave_sum(p[a]);
+            if (lane == 0) {
+                if (first) arena[t.out0 + i * KD + a] = s;
+                else arena[t.out0 + i * KD + a] += s;
+            }
+        }
+    }
+    if (has1) {
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+#pragma unroll
+            for (int b = 0; b < KD; b++) red[k][b][w][lane] = acc1[k][b];
+        __syncthreads();
+        for (int q = threadIdx.x; q < npts * KD; q += blockDim.x) {
+            int pnt = q / KD, b = q - pnt * KD;
+            int k = pnt >> 6, l = pnt & 63;
+            arena[t.out1 + (pbase + pnt) * KD + b] =
+                (red[k][b][0][l] + red[k][b][1][l]) + (red[k][b][2][l] + red[k][b][3][l]);
+        }
+        __syncthreads();
+    }
+}
+
+#define H2P_JIT_MAXNS 4
+template <int KID, int KD>
+__global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ tasks,
+                                                    const double *__restrict__ cx,
+                                                    long long cstride, KParam kp,
+                                                    const double *__restrict__ xin,
+                                                    double *__restrict__ arena) {
+    __shared__ double red[H2P_JIT_MAXNS][KD][4][H2P_WAVE];
+    const BlkTask t = tasks[blockIdx.x];
+    for (int pb = 0; pb < t.n1; pb += H2P_JIT_MAXNS * H2P_WAVE) {
+        const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
+        const int ns = (np + 63) >> 6;
+        const bool first = (pb == 0);
+        if (ns <= 1) blk_jit_pass<KID, KD, 1>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
+        else if (ns <= 2) blk_jit_pass<KID, KD, 2>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
+        else blk_jit_pass<KID, KD, 4>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-side evaluation of AOT blocks (H2P_build_B_AOT / H2P_build_D_AOT,
+// src/H2Pack_build.c:789-889, 1034-1140): one workgroup per block, threads
+// over the row-major entries.
+template <int KID, int KD>
+__global__ void __launch_bounds__(256) k_build_blocks(const BlkTask *__restrict__ tasks,
+                                                      const double *__restrict__ cx,
+                                                      long long cstride, KParam kp,
+                                                      double *__restrict__ blob) {
+    const BlkTask t = tasks[blockIdx.x];
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+    double *A = blob + t.data_off;
+    const long long total = (long long)t.n0 * t.n1;
+    for (long long e = threadIdx.x; e < total; e += blockDim.x) {
+        int i = (int)(e / t.n1), j = (int)(e - (long long)i * t.n1);
+        int gi = t.c0 + i, gj = t.c1 + j;
+        double dx = cx[gi] - cx[gj], dy = cy[gi] - cy[gj], dz = cz[gi] - cz[gj];
+        if constexpr (KD == 1) {
+            A[e] = kval<KID>(dx * dx + dy * dy + dz * dz, kp);
+        } else {
+            double t1, t2, ux, uy, uz;
+            rpy_coef(dx, dy, dz, ca[gi], ca[gj], kp.p1, t1, t2, ux, uy, uz);
+            double u[3] = {ux, uy, uz};
+            const long long ld = (long long)t.n1 * 3;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    A[(long long)(3 * i + a) * ld + 3 * j + b] = t2 * u[a] * u[b] + (a == b ? t1 : 0.0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Direct FP64 n-body rows (examples/direct_nbody.h): grid = (target blocks of
+// 256, source chunks); partial sums per chunk, reduced in a fixed order.
+template <int KID, int KD>
+__global__ void __launch_bounds__(256) k_nbody_partial(const double *__restrict__ src, long long sld, int n_src,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ dst, long long dld, int n_dst,
+                                                       KParam kp, int chunk, double *__restrict__ part) {
+    __shared__ double s[4][256];
+    __shared__ double sxv[3][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int js = blockIdx.y * chunk, je = min(n_src, js + chunk);
+    double tx = 0, ty = 0, tz = 0, ta = 0;
+    if (i < n_dst) {
+        tx = dst[i]; ty = dst[dld + i]; tz = dst[2 * dld + i];
+        if constexpr (KD == 3) ta = dst[3 * dld + i];
+    }
+    double acc[KD];
+#pragma unroll
+    for (int a = 0; a < KD; a++) acc[a] = 0.0;
+    for (int j0 = js; j0 < je; j0 += 256) {
+        int j = j0 + threadIdx.x;
+        __syncthreads();
+        if (j < je) {
+            s[0][threadIdx.x] = src[j]; s[1][threadIdx.x] = src[sld + j]; s[2][threadIdx.x] = src[2 * sld + j];
+            if constexpr (KD == 3) s[3][threadIdx.x] = src[3 * sld + j];
+#pragma unroll
+            for (int b = 0; b < KD; b++) sxv[b][threadIdx.x] = x[(long long)j * KD + b];
+        }
+        __syncthreads();
+        int n = min(256, je - j0);
+        for (int q = 0; q < n; q++) {
+            double dx = tx - s[0][q], dy = ty - s[1][q], dz = tz - s[2][q];
+            if constexpr (KD == 1) {
+                acc[0] += kval<KID>(dx * dx + dy * dy + dz * dz, kp) * sxv[0][q];
+            } else {
+                double t1, t2, ux, uy, uz;
+                rpy_coef(dx, dy, dz, ta, s[3][q], kp.p1, t1, t2, ux, uy, uz);
+                double ud = ux * sxv[0][q] + uy * sxv[1][q] + uz * sxv[2][q];
+                acc[0] += t1 * sxv[0][q] + t2 * ux * ud;
+                acc[1] += t1 * sxv[1][q] + t2 * uy * ud;
+                acc[2] += t1 * sxv[2][q] + t2 * uz * ud;
+            }
+        }
+    }
+    if (i < n_dst)
+#pragma unroll
+        for (int a = 0; a < KD; a++) part[((long long)blockIdx.y * n_dst + i) * KD + a] = acc[a];
+}
+
+__global__ void k_nbody_reduce(const double *__restrict__ part, int nchunk, int len,
+                               double *__restrict__ y) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += part[(long long)c * len + e];
+    y[e] = s;
+}
+
+}  // namespace h2p

@@ -1,0 +1,178 @@
+"""Host-side interface of the MI355X engine.
+
+``H2MatHIP`` takes an H2 representation built on the host
+(``h2pack_amd.construct``; the same fields as the reference's H2Pack struct),
+uploads it once through ``h2p_hip_upload`` and offers the reference's apply
+calls:
+
+* ``matvec(x)``  -- H2P_matvec (src/H2Pack_matvec.c:1162), y := A_H2 x in the
+  original point order, length krnl_mat_size;
+* ``matmul(X, layout)`` -- H2P_matmul (src/H2Pack_matmul.c:455) on an
+  (krnl_mat_size, n_vec) panel, row- or column-major as CBLAS_LAYOUT;
+* device-pointer variants for torch tensors already in HBM.
+
+Failures raise ``H2PHipError`` carrying the library's error code and message
+(the C calls never abort).  There is no CPU path behind these calls.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import abi
+from ._lib import hip_lib
+
+
+class H2PHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"h2p_hip error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise H2PHipError(rc, hip_lib().h2p_hip_last_error().decode(errors="replace"))
+
+
+def kernel_desc(krnl_id: int, param) -> abi.KernelDesc:
+    k = abi.KernelDesc()
+    k.id = int(krnl_id)
+    p = [] if param is None else list(param)
+    k.n_param = len(p)
+    for i, v in enumerate(p[:4]):
+        k.param[i] = float(v)
+    return k
+
+
+def describe(h2, with_blocks: bool = True):
+    """Fill an ``H2PackDesc`` from a construct.H2Pack; returns (desc, keepalive)."""
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    d = abi.H2PackDesc()
+    d.pt_dim, d.xpt_dim, d.krnl_dim = h2.pt_dim, h2.xpt_dim, h2.krnl_dim
+    d.n_point, d.krnl_mat_size = h2.n_point, h2.krnl_mat_size
+    d.n_node, d.n_leaf_node, d.max_child = h2.n_node, h2.n_leaf_node, h2.max_child
+    d.max_level, d.min_adm_level = h2.max_level, h2.min_adm_level
+    for f in ("children", "n_child", "node_level", "pt_cluster", "mat_cluster", "level_n_node",
+              "level_nodes", "r_adm_pairs", "r_inadm_pairs", "node_n_r_adm", "U_nrow", "U_ncol",
+              "fwd_pmt_idx", "bwd_pmt_idx", "B_nrow", "B_ncol", "D_nrow", "D_ncol"):
+        setattr(d, f, arr(getattr(h2, f), np.int32))
+    d.leaf_nodes = arr(h2.leaf_nodes, np.int32)
+    d.n_r_adm_pair, d.n_r_inadm_pair = h2.n_r_adm_pair, h2.n_r_inadm_pair
+    Us = [np.ascontiguousarray(u, dtype=np.float64) for u in h2.U]
+    keep.extend(Us)
+    up = (ct.c_void_p * h2.n_node)(*[u.ctypes.data if u.size else None for u in Us])
+    keep.append(up)
+    d.U = ct.cast(up, ct.c_void_p)
+    jc = [np.ascontiguousarray(j, dtype=np.float64) for j in h2.J_coord]
+    keep.extend(jc)
+    jp = (ct.c_void_p * h2.n_node)(*[j.ctypes.data if j.size else None for j in jc])
+    keep.append(jp)
+    d.J_coord = ct.cast(jp, ct.c_void_p)
+    d.J_npt = arr([j.size for j in h2.J], np.int32)
+    d.coord = arr(h2.coord, np.float64)
+    d.B_ptr = arr(h2.B_ptr, np.uint64)
+    d.D_ptr = arr(h2.D_ptr, np.uint64)
+    if with_blocks and h2.B_data is not None and h2.D_data is not None:
+        d.B_data = arr(h2.B_data, np.float64)
+        d.D_data = arr(h2.D_data, np.float64)
+    else:
+        d.B_data = None
+        d.D_data = None
+    return d, keep
+
+
+class H2MatHIP:
+    """An H2 matrix resident in HBM (one ``h2p_hip_handle``)."""
+
+    def __init__(self, h2, mode: str = "jit", prec: str = "fp64",
+                 host_blocks: bool = True):
+        self.lib = hip_lib()
+        self.h2 = h2
+        self.N = h2.krnl_mat_size
+        self.mode = {"aot": abi.AOT, "jit": abi.JIT}[mode]
+        self.prec = {"fp64": abi.FP64, "fp32mix": abi.FP32MIX}[prec]
+        desc, keep = describe(h2, with_blocks=host_blocks and self.mode == abi.AOT)
+        self._kdesc = kernel_desc(h2.krnl_id, h2.krnl_param)
+        hnd = ct.c_void_p()
+        _check(self.lib.h2p_hip_upload(ct.byref(desc), ct.byref(self._kdesc), self.mode,
+                                       self.prec, ct.byref(hnd)))
+        self.handle = hnd
+        del keep
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.h2p_hip_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- host arrays -----------------------------------------------------
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.N:
+            raise ValueError(f"x has {x.size} entries, expected {self.N}")
+        y = np.empty_like(x)
+        _check(self.lib.h2p_hip_matvec(self.handle, x.ctypes.data, y.ctypes.data))
+        return y
+
+    def matmul(self, X: np.ndarray, layout: str = "col") -> np.ndarray:
+        """Y = A X for X of shape (krnl_mat_size, n_vec)."""
+        if X.shape[0] != self.N:
+            raise ValueError("X must have krnl_mat_size rows")
+        n_vec = X.shape[1]
+        if layout == "row":
+            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            Y = np.empty_like(Xc)
+            _check(self.lib.h2p_hip_matmul(self.handle, abi.ROW_MAJOR, n_vec, Xc.ctypes.data,
+                                           n_vec, Y.ctypes.data, n_vec))
+            return Y
+        Xf = np.asfortranarray(X, dtype=np.float64)
+        Y = np.empty((self.N, n_vec), order="F")
+        _check(self.lib.h2p_hip_matmul(self.handle, abi.COL_MAJOR, n_vec, Xf.ctypes.data,
+                                       self.N, Y.ctypes.data, self.N))
+        return np.ascontiguousarray(Y)
+
+    # -- device arrays (torch tensors on the GPU) --------------------------
+    def matvec_dev(self, x_ptr: int, y_ptr: int, stream: int = 0):
+        _check(self.lib.h2p_hip_matvec_dev(self.handle, ct.c_void_p(x_ptr), ct.c_void_p(y_ptr),
+                                           ct.c_void_p(stream)))
+
+    def matmul_dev(self, layout: int, n_vec: int, x_ptr: int, ldx: int, y_ptr: int, ldy: int,
+                   stream: int = 0):
+        _check(self.lib.h2p_hip_matmul_dev(self.handle, layout, n_vec, ct.c_void_p(x_ptr), ldx,
+                                           ct.c_void_p(y_ptr), ldy, ct.c_void_p(stream)))
+
+    def set_profiling(self, on: bool):
+        _check(self.lib.h2p_hip_set_profiling(self.handle, 1 if on else 0))
+
+    def info(self) -> dict:
+        I = abi.Info()
+        _check(self.lib.h2p_hip_get_info(self.handle, ct.byref(I)))
+        return I.as_dict()
+
+
+def direct_nbody(krnl_id: int, param, coord: np.ndarray, x: np.ndarray,
+                 rows: np.ndarray) -> np.ndarray:
+    """FP64 dense product rows on the device (examples/direct_nbody.h)."""
+    lib = hip_lib()
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    dst = np.ascontiguousarray(coord[:, rows])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    kd = 3 if krnl_id == abi.KRNL_RPY else 1
+    y = np.empty(rows.size * kd)
+    kdsc = kernel_desc(krnl_id, param)
+    _check(lib.h2p_hip_direct_nbody(ct.byref(kdsc), coord.shape[0], coord.ctypes.data,
+                                    coord.shape[1], coord.shape[1], x.ctypes.data,
+                                    dst.ctypes.data, rows.size, rows.size, y.ctypes.data))
+    return y

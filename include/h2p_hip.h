@@ -1,0 +1,162 @@
+/*
+ * h2p_hip.h -- C ABI of the MI355X (gfx950) H2 matrix-vector / matrix-matrix
+ * engine.  Drop-in for the apply half of H2Pack:
+ *
+ *   reference                                         replacement
+ *   void H2P_matvec(H2Pack_p, const DTYPE *x, DTYPE *y)
+ *       src/H2Pack_matvec.h:17, src/H2Pack_matvec.c:1162      h2p_hip_matvec
+ *   void H2P_matmul(H2Pack_p, CBLAS_LAYOUT, int n_vec,
+ *                   const DTYPE *X, int ldx, DTYPE *Y, int ldy)
+ *       src/H2Pack_matmul.h:24-27, src/H2Pack_matmul.c:455     h2p_hip_matmul
+ *   direct_nbody(...)  examples/direct_nbody.h:2-54            h2p_hip_direct_nbody
+ *
+ * The H2 representation is built on the host exactly as before
+ * (H2P_partition_points + proxy points + H2P_build); h2p_hip_upload mirrors
+ * it into HBM once.  The representation is passed as plain arrays
+ * (h2p_hip_h2pack) whose fields carry the meaning of the struct H2Pack
+ * fields of the same names (src/H2Pack_typedef.h:88-207).
+ *
+ * Conventions: every int-returning call returns 0 on success and a negative
+ * H2P_HIP_E* code on failure (h2p_hip_last_error() describes the last
+ * failure of the calling thread); nothing aborts.  A handle is immutable
+ * after upload except for its scratch vectors: calls on one handle must not
+ * overlap, calls on different handles may.  Results are deterministic: no
+ * atomics touch output or intermediate sums.
+ */
+#ifndef H2P_HIP_H
+#define H2P_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kernel ids; parameters as in src/H2Pack_3D_kernels.h */
+#define H2P_HIP_KRNL_COULOMB   0  /* 1/r, K(r=0) = param[0] or 0 (n_param = 0) */
+#define H2P_HIP_KRNL_GAUSSIAN  1  /* exp(-param[0] r^2)                        */
+#define H2P_HIP_KRNL_MATERN32  2  /* (1+sqrt3 l r) exp(-sqrt3 l r), l=param[0] */
+#define H2P_HIP_KRNL_RPY       3  /* 3x3 RPY tensor, param[0] = eta, radius is
+                                     coordinate row 3 (xpt_dim = 4)            */
+
+/* evaluation mode of the B and D blocks (H2Pack's BD_JIT) */
+#define H2P_HIP_AOT 0   /* stored blocks streamed from HBM */
+#define H2P_HIP_JIT 1   /* blocks evaluated in registers from coordinates */
+
+/* arithmetic of the B/D phase */
+#define H2P_HIP_FP64    0  /* everything in FP64 */
+#define H2P_HIP_FP32MIX 1  /* FP32 kernel values and block products with FP32
+                              per-tile sums folded into FP64; U sweeps and the
+                              output accumulation stay FP64 */
+
+/* CBLAS_LAYOUT values */
+#define H2P_HIP_ROW_MAJOR 101
+#define H2P_HIP_COL_MAJOR 102
+
+/* error codes */
+#define H2P_HIP_EINVAL  -1   /* bad argument or inconsistent representation */
+#define H2P_HIP_ENOMEM  -2   /* device or host allocation failed            */
+#define H2P_HIP_EDEVICE -3   /* HIP runtime error                            */
+#define H2P_HIP_ENOSYS  -4   /* combination not supported                    */
+
+typedef struct {
+    int id;            /* H2P_HIP_KRNL_* */
+    int n_param;       /* number of meaningful entries of param */
+    double param[4];
+} h2p_kernel_desc;
+
+/* Fields of struct H2Pack read by the apply path (src/H2Pack_typedef.h).
+ * All arrays are host memory and are only read during h2p_hip_upload. */
+typedef struct {
+    int pt_dim, xpt_dim, krnl_dim, n_point, krnl_mat_size;
+    int n_node, n_leaf_node, max_child, max_level, min_adm_level;
+    const int *children;      /* n_node * max_child, -1 padded            */
+    const int *n_child;       /* n_node                                   */
+    const int *node_level;    /* n_node                                   */
+    const int *pt_cluster;    /* 2 * n_node, inclusive point ranges        */
+    const int *mat_cluster;   /* 2 * n_node, inclusive vector ranges       */
+    const int *level_n_node;  /* max_level + 1                            */
+    const int *level_nodes;   /* (max_level + 1) * n_leaf_node            */
+    const int *leaf_nodes;    /* n_leaf_node: height_nodes row 0 (D order) */
+    int n_r_adm_pair, n_r_inadm_pair;
+    const int *r_adm_pairs;   /* 2 * n_r_adm_pair                         */
+    const int *r_inadm_pairs; /* 2 * n_r_inadm_pair                       */
+    const int *node_n_r_adm;  /* n_node                                   */
+    const int *U_nrow, *U_ncol;        /* n_node                          */
+    const double *const *U;            /* n_node row-major U[i]->data     */
+    const int *J_npt;                  /* n_node, skeleton point counts   */
+    const double *const *J_coord;      /* n_node, xpt_dim x J_npt          */
+    const double *coord;               /* xpt_dim x n_point, sorted        */
+    const int *fwd_pmt_idx, *bwd_pmt_idx;  /* krnl_mat_size              */
+    const int *B_nrow, *B_ncol;        /* n_r_adm_pair                     */
+    const size_t *B_ptr;               /* n_r_adm_pair + 1                 */
+    const double *B_data;              /* NULL: evaluated on the device    */
+    const int *D_nrow, *D_ncol;        /* n_leaf_node + n_r_inadm_pair     */
+    const size_t *D_ptr;               /* n_leaf_node + n_r_inadm_pair + 1 */
+    const double *D_data;              /* NULL: evaluated on the device    */
+} h2p_hip_h2pack;
+
+typedef struct h2p_hip_handle h2p_hip_handle;
+
+/* Statistics mirroring H2Pack's mat_size / JIT_flops / timers model
+ * (src/H2Pack_typedef.h:210-246, src/H2Pack_typedef.c:447-514). */
+typedef struct {
+    size_t U_elems, B_elems, D_elems;   /* stored FP64 elements            */
+    size_t hbm_bytes_model;             /* 8*(2 U + B + D) + vector terms  */
+    double B_pairs, D_pairs;            /* kernel evaluations per matvec   */
+    double jit_flops_model;             /* krnl_bimv_flops * pairs         */
+    size_t device_bytes;                /* HBM held by the handle          */
+    int n_block_tasks, n_levels;
+    int mode, prec;
+    /* last profiled call, milliseconds (0 when profiling is off) */
+    double t_fwd, t_blocks, t_reduce, t_bwd, t_final, t_total;
+} h2p_hip_info;
+
+/* Mirror an H2 representation into HBM.  mode = H2P_HIP_AOT / _JIT,
+ * prec = H2P_HIP_FP64 / _FP32MIX.  In AOT mode, NULL B_data / D_data ask
+ * the device to evaluate the blocks itself (same values as H2P_build_B_AOT /
+ * H2P_build_D_AOT, src/H2Pack_build.c:789-889, 1034-1140). */
+int h2p_hip_upload(const h2p_hip_h2pack *h2, const h2p_kernel_desc *krnl,
+                   int mode, int prec, h2p_hip_handle **out);
+
+/* y := A_H2 x, host pointers, length krnl_mat_size, original point order;
+ * synchronous; same semantics as H2P_matvec. */
+int h2p_hip_matvec(h2p_hip_handle *h, const double *x, double *y);
+
+/* Same on device pointers, enqueued on `stream` (a hipStream_t, NULL =
+ * default stream); returns once enqueued. */
+int h2p_hip_matvec_dev(h2p_hip_handle *h, const double *d_x, double *d_y,
+                       void *stream);
+
+/* Y := A_H2 X for n_vec columns; layout H2P_HIP_ROW_MAJOR (ldx >= n_vec) or
+ * H2P_HIP_COL_MAJOR (ldx >= krnl_mat_size); host pointers; synchronous; same
+ * semantics as H2P_matmul. */
+int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec,
+                   const double *X, int ldx, double *Y, int ldy);
+
+/* Same on device pointers, enqueued on `stream`. */
+int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
+                       const double *d_X, int ldx, double *d_Y, int ldy,
+                       void *stream);
+
+/* Record per-phase HIP events on each call (adds synchronisation). */
+int h2p_hip_set_profiling(h2p_hip_handle *h, int on);
+
+int h2p_hip_get_info(const h2p_hip_handle *h, h2p_hip_info *info);
+
+void h2p_hip_free(h2p_hip_handle *h);
+
+const char *h2p_hip_last_error(void);
+
+/* FP64 dense kernel product on the device: y[i] = sum_j K(dst_i, src_j) x_j
+ * for n_dst target points (host pointers, coordinates xpt_dim x ld).  The
+ * accuracy reference of examples/direct_nbody.h:2-54. */
+int h2p_hip_direct_nbody(const h2p_kernel_desc *krnl, int xpt_dim,
+                         const double *src, int src_ld, int n_src,
+                         const double *x,
+                         const double *dst, int dst_ld, int n_dst, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H2P_HIP_H */

@@ -1,0 +1,153 @@
+"""Pinning the oracle (oracle/h2oracle.c) before trusting it.
+
+The reference cannot be compiled in this image and holds no golden vectors
+for the apply path, so the oracle is pinned by what the mathematics fixes:
+
+* the H2 matrix assembled explicitly from nested bases (tests/problems.py
+  ``dense_h2``, independent numpy code) -- catches dropped, transposed or
+  mis-indexed U / B / D terms in every phase;
+* closed forms of the four kernels (Coulomb, Gaussian, Matern-3/2 and the
+  Rotne-Prager-Yamakawa tensor in its textbook form for equal and unequal
+  radii, its continuity at r = a_i + a_j);
+* the FP64 direct n-body product against the build tolerance
+  (extra/test_H2_scalar.c:89-99);
+* matmul == matvec per column (extra/test_H2_matmul.h:18-116), and AOT == JIT.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from h2pack_amd import construct as C
+from tests.problems import dense_h2, problem, rel
+
+
+@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "tiny_rpy"])
+@pytest.mark.parametrize("jit", [0, 1])
+def test_oracle_matvec_equals_dense_assembly(name, jit):
+    h2, _ = problem(name)
+    A = dense_h2(h2)
+    x = np.random.default_rng(1).random(h2.krnl_mat_size) - 0.5
+    y = O.OracleH2(h2, BD_JIT=jit).matvec(x, n_thread=4)
+    assert rel(y, A @ x) < 1e-13
+
+
+def test_dense_assembly_catches_planted_mistakes():
+    """Mistakes planted in the representation the oracle consumes (a B block
+    used transposed, a dropped D block, a sign error in one U entry) are all
+    visible to the dense-assembly check, so the check can see the kinds of
+    error an apply path makes."""
+    import copy
+    h2, _ = problem("tiny_coulomb")
+    A = dense_h2(h2)
+    x = np.random.default_rng(2).random(h2.krnl_mat_size) - 0.5
+    y_ok = O.OracleH2(h2, BD_JIT=0).matvec(x, n_thread=2)
+    assert rel(y_ok, A @ x) < 1e-13
+    # (1) a square B block stored transposed
+    sq = np.nonzero(h2.B_nrow == h2.B_ncol)[0]
+    i = int(sq[0])
+    bad = copy.copy(h2)
+    B = h2.B_data.copy()
+    s, e, r = int(h2.B_ptr[i]), int(h2.B_ptr[i + 1]), int(h2.B_nrow[i])
+    B[s:e] = B[s:e].reshape(r, r).T.reshape(-1)
+    bad.B_data = B
+    assert rel(O.OracleH2(bad, BD_JIT=0).matvec(x, n_thread=2), A @ x) > 1e-8
+    # (2) a dropped off-diagonal D block
+    bad = copy.copy(h2)
+    D = h2.D_data.copy()
+    D[int(h2.D_ptr[h2.n_leaf_node]):int(h2.D_ptr[h2.n_leaf_node + 1])] = 0.0
+    bad.D_data = D
+    assert rel(O.OracleH2(bad, BD_JIT=0).matvec(x, n_thread=2), A @ x) > 1e-6
+    # (3) a sign error in one non-skeleton U entry of a leaf
+    bad = copy.copy(h2)
+    bad.U = [u.copy() for u in h2.U]
+    leaf = next(n for n in h2.leaf_nodes if h2.U[n].shape[1] > 0)
+    k = int(np.argmax(np.abs(bad.U[leaf]).sum(axis=1) * (np.abs(bad.U[leaf]).max(axis=1) < 0.999)))
+    bad.U[leaf][k] *= -1.0
+    assert rel(O.OracleH2(bad, BD_JIT=0).matvec(x, n_thread=2), A @ x) > 1e-8
+
+
+def _rpy_textbook_equal(r_vec, a, eta):
+    """RPY mobility for equal radii (Rotne & Prager 1969, Yamakawa 1970)."""
+    r = np.linalg.norm(r_vec)
+    if r == 0:
+        return np.eye(3) / (6 * math.pi * eta * a)
+    u = r_vec / r
+    if r >= 2 * a:
+        pre = 1.0 / (8 * math.pi * eta * r)
+        return pre * ((1 + 2 * a * a / (3 * r * r)) * np.eye(3) + (1 - 2 * a * a / (r * r)) * np.outer(u, u))
+    pre = 1.0 / (6 * math.pi * eta * a)
+    return pre * ((1 - 9 * r / (32 * a)) * np.eye(3) + (3 * r / (32 * a)) * np.outer(u, u))
+
+
+@pytest.mark.parametrize("r", [0.0, 0.7, 1.5, 2.0000001, 3.3, 10.0])
+def test_oracle_rpy_equals_textbook(r):
+    eta, a = 1.3, 1.0
+    d = np.array([0.3, -0.4, 0.866]) / np.linalg.norm([0.3, -0.4, 0.866])
+    c0 = np.array([[1.0], [2.0], [3.0], [a]])
+    c1 = c0.copy()
+    c1[:3, 0] -= r * d
+    K = O.kernel_eval(3, [eta], c0, c1)
+    assert np.allclose(K, _rpy_textbook_equal(r * d, a, eta), rtol=1e-12, atol=1e-14)
+
+
+def test_oracle_rpy_unequal_radii_regimes():
+    eta = 0.8
+    C0 = 1.0 / (6 * math.pi * eta)
+    # one sphere inside the other: C / max(a) I (Zuk et al. 2014)
+    c0 = np.array([[0.0], [0.0], [0.0], [2.0]])
+    c1 = np.array([[0.5], [0.0], [0.0], [0.5]])
+    assert np.allclose(O.kernel_eval(3, [eta], c0, c1), C0 / 2.0 * np.eye(3))
+    # continuity where the far and overlapping formulas meet, r = a_i + a_j
+    for eps in (1e-9, -1e-9):
+        c1 = np.array([[2.5 + eps], [0.0], [0.0], [0.5]])
+        K = O.kernel_eval(3, [eta], c0, c1)
+        c1b = np.array([[2.5 - eps], [0.0], [0.0], [0.5]])
+        Kb = O.kernel_eval(3, [eta], c0, c1b)
+        assert np.allclose(K, Kb, rtol=1e-6)
+    # symmetry of the tensor kernel: K(x, y) = K(y, x)^T
+    rng = np.random.default_rng(3)
+    p = np.concatenate([rng.random((3, 7)) * 4, 0.5 + rng.random((1, 7))], axis=0)
+    q = np.concatenate([rng.random((3, 5)) * 4, 0.5 + rng.random((1, 5))], axis=0)
+    assert np.allclose(O.kernel_eval(3, [eta], p, q), O.kernel_eval(3, [eta], q, p).T, rtol=1e-13)
+
+
+def test_oracle_scalar_kernels_closed_forms():
+    c0 = np.array([[0.0], [0.0], [0.0]])
+    c1 = np.array([[0.0, 1.0, 0.0], [0.0, 2.0, 0.0], [0.0, 2.0, 0.5]])   # r = 0, 3, 0.5
+    r = np.array([0.0, 3.0, 0.5])
+    assert np.allclose(O.kernel_eval(0, [], c0, c1), [[0.0, 1 / 3, 2.0]])
+    assert np.allclose(O.kernel_eval(0, [4.0], c0, c1), [[4.0, 1 / 3, 2.0]])
+    assert np.allclose(O.kernel_eval(1, [0.3], c0, c1), [np.exp(-0.3 * r * r)])
+    t = math.sqrt(3) * 0.9 * r
+    assert np.allclose(O.kernel_eval(2, [0.9], c0, c1), [(1 + t) * np.exp(-t)])
+
+
+@pytest.mark.parametrize("name", ["coulomb", "gaussian", "matern32", "rpy"])
+def test_oracle_matches_direct_nbody_to_build_tolerance(name):
+    h2, coord = problem(name)
+    kid, param = h2.krnl_id, list(h2.krnl_param)
+    rng = np.random.default_rng(7)
+    x = rng.random(h2.krnl_mat_size) - 0.5
+    rows = np.sort(rng.choice(h2.n_point, 300, replace=False))
+    y_ref = O.direct_nbody(kid, param, coord, x, rows, n_thread=4)
+    # the direct product is itself checked against a numpy kernel matrix
+    K = C.kernel_eval(kid, param, coord[:, rows], coord)
+    assert rel(K @ x, y_ref) < 1e-13
+    y = O.OracleH2(h2, BD_JIT=1).matvec(x, n_thread=4)
+    kd = h2.krnl_dim
+    yr = y.reshape(-1, kd)[rows].reshape(-1)
+    assert rel(yr, y_ref) < 20 * h2.QR_stop_tol
+
+
+@pytest.mark.parametrize("name", ["coulomb", "rpy"])
+def test_oracle_aot_equals_jit_and_matmul_equals_matvec(name):
+    h2, _ = problem(name)
+    rng = np.random.default_rng(9)
+    X = rng.random((h2.krnl_mat_size, 6)) - 0.5
+    oa, oj = O.OracleH2(h2, BD_JIT=0), O.OracleH2(h2, BD_JIT=1)
+    cols = np.stack([oa.matvec(X[:, i], n_thread=4) for i in range(6)], axis=1)
+    assert rel(np.stack([oj.matvec(X[:, i], n_thread=4) for i in range(6)], 1), cols) < 1e-14
+    assert rel(oa.matmul(X, "col", n_thread=4), cols) < 1e-14
+    assert rel(oj.matmul(X, "row", n_thread=4), cols) < 1e-14
