@@ -94,9 +94,12 @@ struct h2p_hip_handle {
     long long arena_len = 0;
     size_t device_bytes = 0;
     h2p_hip_info info{};
+    // per-phase timing: one set of 6 events per profiled call, read back and
+    // averaged by h2p_hip_get_info, so the profiled calls do not synchronise
     int profiling = 0;
-    hipEvent_t ev[7]{};
-    bool ev_ok = false;
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;      // calls recorded since the last read
+    int ev_cap = 0;       // calls the pool holds
     hipStream_t own_stream = nullptr;
 };
 
@@ -107,8 +110,7 @@ static void free_handle(h2p_hip_handle *h) {
                     h->d_contrib, h->d_mm_x, h->d_mm_y};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    if (h->ev_ok)
-        for (auto &e : h->ev) (void)hipEventDestroy(e);
+    for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
 }
@@ -145,39 +147,32 @@ static void dispatch_build(h2p_hip_handle *h, hipStream_t s) {
 
 // One matvec on device pointers (x, y in the caller's order).
 static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
-    const bool prof = h->profiling && h->ev_ok;
-    if (prof) HIP_TRY(hipEventRecord(h->ev[0], s));
+    const bool prof = h->profiling && h->ev_used < h->ev_cap;
+    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * 6] : nullptr;
+    if (prof) HIP_TRY(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
     for (const Level &L : h->fwd_levels)
         hipLaunchKernelGGL(k_fwd, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
-    if (prof) HIP_TRY(hipEventRecord(h->ev[1], s));
+    if (prof) HIP_TRY(hipEventRecord(ev[1], s));
     if (h->n_blk > 0) {
         if (h->mode == H2P_HIP_AOT)
             hipLaunchKernelGGL(k_blocks_aot, dim3(h->n_blk), dim3(256), 0, s, h->d_blk, h->d_blob, h->d_xin, h->d_arena);
         else
             dispatch_jit(h, s);
     }
-    if (prof) HIP_TRY(hipEventRecord(h->ev[2], s));
+    if (prof) HIP_TRY(hipEventRecord(ev[2], s));
     if (h->n_red > 0)
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
-    if (prof) HIP_TRY(hipEventRecord(h->ev[3], s));
+    if (prof) HIP_TRY(hipEventRecord(ev[3], s));
     for (const Level &L : h->bwd_levels)
         hipLaunchKernelGGL(k_bwd, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1);
-    if (prof) HIP_TRY(hipEventRecord(h->ev[4], s));
+    if (prof) HIP_TRY(hipEventRecord(ev[4], s));
     hipLaunchKernelGGL(k_leaf_final, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
                        h->d_UT, h->d_y1, h->d_fwd, d_y);
     HIP_TRY(hipGetLastError());
     if (prof) {
-        HIP_TRY(hipEventRecord(h->ev[5], s));
-        HIP_TRY(hipEventSynchronize(h->ev[5]));
-        float t[5];
-        for (int i = 0; i < 5; i++) HIP_TRY(hipEventElapsedTime(&t[i], h->ev[i], h->ev[i + 1]));
-        h->info.t_fwd = t[0];
-        h->info.t_blocks = t[1];
-        h->info.t_reduce = t[2];
-        h->info.t_bwd = t[3];
-        h->info.t_final = t[4];
-        h->info.t_total = t[0] + t[1] + t[2] + t[3] + t[4];
+        HIP_TRY(hipEventRecord(ev[5], s));
+        h->ev_used++;
     }
     return 0;
 }
@@ -533,11 +528,6 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
             return bail(fail(H2P_HIP_EINVAL, "give both B_data and D_data or neither"));
         }
     }
-    if (hipEventCreate(&h->ev[0]) == hipSuccess) {
-        h->ev_ok = true;
-        for (int i = 1; i < 7; i++)
-            if (hipEventCreate(&h->ev[i]) != hipSuccess) h->ev_ok = false;
-    }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "stream create failed"));
     // ---- statistics (H2Pack's mat_size / JIT_flops model)
     h2p_hip_info &I = h->info;
@@ -658,11 +648,41 @@ int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, in
 int h2p_hip_set_profiling(h2p_hip_handle *h, int on) {
     if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
     h->profiling = on ? 1 : 0;
+    h->ev_used = 0;
+    if (on && h->ev_cap == 0) {
+        const int cap = 512;
+        h->ev_pool.resize((size_t)cap * 6);
+        for (auto &e : h->ev_pool)
+            if (hipEventCreate(&e) != hipSuccess) return fail(H2P_HIP_EDEVICE, "hipEventCreate failed");
+        h->ev_cap = cap;
+    }
     return 0;
 }
 
-int h2p_hip_get_info(const h2p_hip_handle *h, h2p_hip_info *info) {
+int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info) {
     if (!h || !info) return fail(H2P_HIP_EINVAL, "NULL argument");
+    if (h->ev_used > 0) {
+        // average the phases of the calls profiled since the last read
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int c = 0; c < h->ev_used; c++) {
+            hipEvent_t *ev = &h->ev_pool[(size_t)c * 6];
+            HIP_TRY(hipEventSynchronize(ev[5]));
+            for (int i = 0; i < 5; i++) {
+                float t = 0.f;
+                HIP_TRY(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+                acc[i] += t;
+            }
+        }
+        const double n = h->ev_used;
+        h->info.t_fwd = acc[0] / n;
+        h->info.t_blocks = acc[1] / n;
+        h->info.t_reduce = acc[2] / n;
+        h->info.t_bwd = acc[3] / n;
+        h->info.t_final = acc[4] / n;
+        h->info.t_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / n;
+        h->info.n_profiled = h->ev_used;
+        h->ev_used = 0;
+    }
     *info = h->info;
     return 0;
 }

@@ -108,8 +108,10 @@ typedef struct {
     size_t device_bytes;                /* HBM held by the handle          */
     int n_block_tasks, n_levels;
     int mode, prec;
-    /* last profiled call, milliseconds (0 when profiling is off) */
+    /* mean per-phase device time in milliseconds over the calls profiled
+     * since the previous h2p_hip_get_info (HIP events on the call's stream) */
     double t_fwd, t_blocks, t_reduce, t_bwd, t_final, t_total;
+    int n_profiled;
 } h2p_hip_info;
 
 /* Mirror an H2 representation into HBM.  mode = H2P_HIP_AOT / _JIT,
@@ -139,10 +141,13 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
                        const double *d_X, int ldx, double *d_Y, int ldy,
                        void *stream);
 
-/* Record per-phase HIP events on each call (adds synchronisation). */
+/* Record per-phase HIP events on each call (up to 512 calls between reads;
+ * recording does not synchronise). */
 int h2p_hip_set_profiling(h2p_hip_handle *h, int on);
 
-int h2p_hip_get_info(const h2p_hip_handle *h, h2p_hip_info *info);
+/* Sizes, the byte/flop model and, after profiled calls, the mean per-phase
+ * times (waits for the profiled calls to finish). */
+int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info);
 
 void h2p_hip_free(h2p_hip_handle *h);
 
