@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--check-rows", type=int, default=1000)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run one 2 GiB device-to-device copy first, the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -78,6 +81,13 @@ def main():
 
     N, K, W = args.n, args.steps, args.warmup
     sigma2 = 1e-2
+    if args.copy_calib:
+        src = torch.ones(2 << 27, dtype=torch.float64, device="cuda:0")   # 2 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 2 GiB read + 2 GiB written by one copy kernel")
+        del src, dst
     t0 = time.perf_counter()
     coord = C.uniform_cube_points(N, 10.0, args.seed)
     h2 = C.build_h2(coord, C.KRNL_GAUSSIAN, [1.0], 1e-6, 300, BD_JIT=1, pp_kind="numerical",
@@ -119,7 +129,6 @@ def main():
     mj = H2MatHIP(h2, mode="jit")
     log(f"[bench] JIT upload {time.perf_counter() - t:.1f}s")
     el_j, info_j, y_j = timed(mj, "fp64-jit")
-    res = {}
     # ---- accuracy: direct FP64 n-body on sampled rows (device utility)
     rows = np.sort(rng.choice(N, args.check_rows, replace=False))
     y_dir = direct_nbody(C.KRNL_GAUSSIAN, [1.0], coord, x, rows)
@@ -127,6 +136,22 @@ def main():
     log(f"[bench] fp64-jit rel err vs direct n-body ({args.check_rows} rows) = {err_direct:.3e}")
     mj.close()
     del mj
+
+    # ---- FP32-mixed JIT
+    t = time.perf_counter()
+    mf = H2MatHIP(h2, mode="jit", prec="fp32mix")
+    el_f, info_f, y_f = timed(mf, "fp32mix-jit")
+    mf.close()
+    del mf
+    err_f_direct = float(np.linalg.norm(y_f[rows] - y_dir) / np.linalg.norm(y_dir))
+    err_f_fp64 = float(np.linalg.norm(y_f - y_j) / np.linalg.norm(y_j))
+    fp32 = {"matvecs_per_s": K / el_f, "ms_per_step": el_f / K * 1e3,
+            "speedup_vs_fp64": el_j / el_f,
+            "rel_err_vs_fp64_gpu": err_f_fp64,
+            "rel_err_vs_direct_nbody_1000rows": err_f_direct,
+            "kernel_ms": info_f["t_blocks"]}
+    log(f"[bench] fp32mix-jit: {K / el_f:.2f} matvec/s ({el_j / el_f:.2f}x FP64); rel err vs FP64 GPU "
+        f"{err_f_fp64:.3e}, vs direct n-body {err_f_direct:.3e}")
 
     # ---- AOT (stored blocks evaluated into HBM at upload)
     aot = None
@@ -188,6 +213,7 @@ def main():
         "cpu_baseline": cpu,
         "rel_err_vs_direct_nbody_1000rows": err_direct,
         "rel_err_vs_oracle": err_oracle,
+        "fp32mix": fp32,
         "aot": aot,
         "phases_ms": {k: info_j[k] for k in ("t_fwd", "t_blocks", "t_reduce", "t_bwd", "t_final", "t_total")},
         "h2": {"build_s": t_build, "levels": h2.max_level + 1, "leaves": h2.n_leaf_node,

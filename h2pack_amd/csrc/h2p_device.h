@@ -74,6 +74,83 @@ __device__ __forceinline__ void rpy_coef(double dx, double dy, double dz,
     }
 }
 
+// 2^x for x <= 0 in FP64: x = n + f with n = rint(x), f in [-1/2, 1/2]
+// (exact, the reduction is in base 2), 2^f = e^(f ln 2) by its Taylor series
+// through degree 12 (truncation below 2.5e-16 relative), then scaled by 2^n
+// with ldexp, which returns 0 once n drops below the subnormal range.
+__device__ __forceinline__ double exp2_nonpos(double x) {
+    const double n = __builtin_rint(x);
+    const double f = x - n;
+    double p = 2.5678435993488206e-11;
+    p = __builtin_fma(p, f, 4.4455382718708116e-10);
+    p = __builtin_fma(p, f, 7.054911620801123e-09);
+    p = __builtin_fma(p, f, 1.01780860092397e-07);
+    p = __builtin_fma(p, f, 1.321548679014431e-06);
+    p = __builtin_fma(p, f, 1.5252733804059841e-05);
+    p = __builtin_fma(p, f, 0.0001540353039338161);
+    p = __builtin_fma(p, f, 0.0013333558146428443);
+    p = __builtin_fma(p, f, 0.009618129107628477);
+    p = __builtin_fma(p, f, 0.05550410866482158);
+    p = __builtin_fma(p, f, 0.24022650695910072);
+    p = __builtin_fma(p, f, 0.6931471805599453);
+    p = __builtin_fma(p, f, 1.0);
+    return __builtin_amdgcn_ldexp(p, (int)n);
+}
+
+// Scalar kernels on prepared squared distances.  Coordinates are scaled
+// before the distance is formed so that the kernels need no parameter:
+//   Gaussian exp(-l r^2)          : scale sqrt(l log2 e)   -> K = 2^(-q)
+//   Matern32 (1+t) e^{-t}, t=sqrt3 l r : scale sqrt3 l     -> K = (1+s) 2^(-s log2 e), s = sqrt(q)
+//   Coulomb 1/r                   : scale 1                -> K = q == 0 ? diag : rsqrt(q)
+template <int KID>
+__device__ __forceinline__ double kprep(double q, double diag) {
+    if constexpr (KID == KC) {
+        return (q == 0.0) ? diag : rsqrt(q);
+    } else if constexpr (KID == KG) {
+        return exp2_nonpos(-q);
+    } else {
+        const double s = __builtin_sqrt(q);
+        return (1.0 + s) * exp2_nonpos(-1.4426950408889634 * s);
+    }
+}
+
+template <int KID>
+__device__ __forceinline__ float kprep(float q, float diag) {
+    if constexpr (KID == KC) {
+        return (q == 0.0f) ? diag : __builtin_amdgcn_rsqf(q);
+    } else if constexpr (KID == KG) {
+        return __builtin_amdgcn_exp2f(-q);
+    } else {
+        const float s = __builtin_sqrtf(q);
+        return (1.0f + s) * __builtin_amdgcn_exp2f(-1.4426950408889634f * s);
+    }
+}
+
+// Coordinate scale of the prepared form above.
+__host__ __device__ inline double kscale(int kid, double p0) {
+    if (kid == KG) return __builtin_sqrt(p0 * 1.4426950408889634);
+    if (kid == KM) return 1.7320508075688772 * p0;
+    return 1.0;
+}
+
+// Sum over the 64 lanes of four per-lane values at once (butterfly
+// reduce-scatter): 7 exchanges instead of 4 x 6.  On return lane l holds
+// the full sum of value (l >> 4) & 3.
+template <typename T>
+__device__ __forceinline__ T wave_sum4(T v0, T v1, T v2, T v3) {
+    const int lane = threadIdx.x & 63;
+    const bool b5 = lane & 32, b4 = lane & 16;
+    T s0 = b5 ? v0 : v2, s1 = b5 ? v1 : v3;
+    T k0 = b5 ? v2 : v0, k1 = b5 ? v3 : v1;
+    T a0 = k0 + __shfl_xor(s0, 32, H2P_WAVE);
+    T a1 = k1 + __shfl_xor(s1, 32, H2P_WAVE);
+    T s = b4 ? a0 : a1, k = b4 ? a1 : a0;
+    T c = k + __shfl_xor(s, 16, H2P_WAVE);
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) c += __shfl_xor(c, m, H2P_WAVE);
+    return c;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, H2P_WAVE);

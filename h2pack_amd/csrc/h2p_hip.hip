@@ -127,11 +127,18 @@ static void launch_build(h2p_hip_handle *h, hipStream_t s) {
                        h->d_coord, h->CT, h->kp, h->d_blob);
 }
 
+template <int KID, typename T>
+static void launch_jit2(h2p_hip_handle *h, hipStream_t s) {
+    hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                       h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+}
+
 static void dispatch_jit(h2p_hip_handle *h, hipStream_t s) {
+    const bool f32 = h->prec == H2P_HIP_FP32MIX;
     switch (h->kid) {
-        case KC: launch_jit<KC, 1>(h, s); break;
-        case KG: launch_jit<KG, 1>(h, s); break;
-        case KM: launch_jit<KM, 1>(h, s); break;
+        case KC: f32 ? launch_jit2<KC, float>(h, s) : launch_jit2<KC, double>(h, s); break;
+        case KG: f32 ? launch_jit2<KG, float>(h, s) : launch_jit2<KG, double>(h, s); break;
+        case KM: f32 ? launch_jit2<KM, float>(h, s) : launch_jit2<KM, double>(h, s); break;
         default: launch_jit<KR, 3>(h, s); break;
     }
 }
@@ -204,7 +211,9 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     *out = nullptr;
     if (!g || !krnl) return fail(H2P_HIP_EINVAL, "NULL representation or kernel");
     if (mode != H2P_HIP_AOT && mode != H2P_HIP_JIT) return fail(H2P_HIP_EINVAL, "bad mode");
-    if (prec != H2P_HIP_FP64) return fail(H2P_HIP_ENOSYS, "only FP64 is available in this build");
+    if (prec != H2P_HIP_FP64 && prec != H2P_HIP_FP32MIX) return fail(H2P_HIP_EINVAL, "bad precision");
+    if (prec == H2P_HIP_FP32MIX && (mode != H2P_HIP_JIT || krnl->id == KR))
+        return fail(H2P_HIP_ENOSYS, "FP32-mixed is available for JIT mode with the scalar kernels");
     if (krnl->id < 0 || krnl->id > 3) return fail(H2P_HIP_EINVAL, "bad kernel id");
     const int kd = g->krnl_dim;
     if (g->pt_dim != 3) return fail(H2P_HIP_EINVAL, "pt_dim must be 3");

@@ -326,6 +326,144 @@ __global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Grouped kernel-evaluating bi-matvec for the scalar kernels, FP64 (T =
+// double) or FP32-mixed (T = float).
+//
+//  * Source points (columns) live in lanes, NS strips of 64 per pass.
+//  * Target points (rows) are staged in LDS, already prepared (scaled, and
+//    for FP32 shifted to the block's first target so the FP32 coordinates are
+//    block-local); each wave takes four rows per iteration and reduces the
+//    four row sums with one butterfly.
+//  * Lanes without a source point use a valid point with x = 0, so every
+//    kernel value stays finite and no per-pair select is needed.
+//  * FP32 mode: kernel values and products in FP32; row sums leave each
+//    pass as FP32 and are folded into the FP64 partial outputs; the column
+//    sums are FP32 over tiles of 16 rows, folded into FP64 registers.
+#define H2P_JIT_ROWS 256                 // target rows staged per chunk
+template <typename T>
+struct JitRow { T x, y, z, v; };
+
+template <int KID, typename T, int NS>
+__device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
+                                          const double *__restrict__ cx, long long cstride,
+                                          double scale, double diag, double ox, double oy, double oz,
+                                          const double *__restrict__ xin, double *__restrict__ arena,
+                                          JitRow<T> *rows, double (*red)[4][H2P_WAVE], bool first) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride;
+    const T dg = (T)diag;
+    T sx[NS], sy[NS], sz[NS], x0[NS];
+    double acc1d[NS];
+    T acc1[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        const int p = k * H2P_WAVE + lane;
+        const bool ok = p < npts;
+        const int g = t.c1 + pbase + (ok ? p : 0);
+        sx[k] = (T)((cx[g] - ox) * scale);
+        sy[k] = (T)((cy[g] - oy) * scale);
+        sz[k] = (T)((cz[g] - oz) * scale);
+        x0[k] = ok ? (T)xin[t.in0 + pbase + p] : (T)0;
+        acc1[k] = (T)0;
+        acc1d[k] = 0.0;
+    }
+    const bool has1 = t.flags & 1;
+    for (int rc = 0; rc < t.n0; rc += H2P_JIT_ROWS) {
+        const int nr = min(t.n0 - rc, H2P_JIT_ROWS);
+        const int nr_pad = (nr + 15) & ~15;
+        __syncthreads();
+        for (int q = threadIdx.x; q < nr_pad; q += blockDim.x) {
+            const bool ok = q < nr;
+            const int g = t.c0 + rc + (ok ? q : 0);
+            JitRow<T> r;
+            r.x = (T)((cx[g] - ox) * scale);
+            r.y = (T)((cy[g] - oy) * scale);
+            r.z = (T)((cz[g] - oz) * scale);
+            r.v = (ok && has1) ? (T)xin[t.in1 + rc + q] : (T)0;
+            rows[q] = r;
+        }
+        __syncthreads();
+        int tile = 0;
+        for (int r0 = 4 * w; r0 < nr_pad; r0 += 16) {
+            T p[4];
+            JitRow<T> rr[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                rr[a] = rows[r0 + a];
+                p[a] = (T)0;
+            }
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const T dx = rr[a].x - sx[k], dy = rr[a].y - sy[k], dz = rr[a].z - sz[k];
+                    const T q = dx * dx + dy * dy + dz * dz;
+                    const T kv = kprep<KID>(q, dg);
+                    p[a] += kv * x0[k];
+                    acc1[k] += kv * rr[a].v;
+                }
+            }
+            const T s = wave_sum4<T>(p[0], p[1], p[2], p[3]);
+            const int myrow = r0 + ((lane >> 4) & 3);
+            if ((lane & 15) == 0 && myrow < nr) {
+                double *o = arena + t.out0 + rc + myrow;
+                if (first) *o = (double)s;
+                else *o += (double)s;
+            }
+            if constexpr (sizeof(T) == 4) {
+                if (++tile == 4) {   // fold the FP32 column sums of 16 rows
+                    tile = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; k++) {
+                        acc1d[k] += (double)acc1[k];
+                        acc1[k] = (T)0;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc1d[k] += (double)acc1[k];
+    if (has1) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NS; k++) red[k][w][lane] = acc1d[k];
+        __syncthreads();
+        for (int c = threadIdx.x; c < npts; c += blockDim.x) {
+            const int k = c >> 6, l = c & 63;
+            arena[t.out1 + pbase + c] = (red[k][0][l] + red[k][1][l]) + (red[k][2][l] + red[k][3][l]);
+        }
+    }
+}
+
+template <int KID, typename T>
+__global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks,
+                                                     const double *__restrict__ cx,
+                                                     long long cstride, double scale, double diag,
+                                                     const double *__restrict__ xin,
+                                                     double *__restrict__ arena) {
+    __shared__ JitRow<T> rows[H2P_JIT_ROWS];
+    __shared__ double red[H2P_JIT_MAXNS][4][H2P_WAVE];
+    const BlkTask t = tasks[blockIdx.x];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if constexpr (sizeof(T) == 4) {   // block-local origin for FP32 coordinates
+        ox = cx[t.c0];
+        oy = cx[cstride + t.c0];
+        oz = cx[2 * cstride + t.c0];
+    }
+    for (int pb = 0; pb < t.n1; pb += H2P_JIT_MAXNS * H2P_WAVE) {
+        const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
+        const int ns = (np + 63) >> 6;
+        const bool first = (pb == 0);
+        if (ns <= 1) jit2_pass<KID, T, 1>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
+        else if (ns <= 2) jit2_pass<KID, T, 2>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
+        else if (ns <= 3) jit2_pass<KID, T, 3>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
+        else jit2_pass<KID, T, 4>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Device-side evaluation of AOT blocks (H2P_build_B_AOT / H2P_build_D_AOT,
 // src/H2Pack_build.c:789-889, 1034-1140): one workgroup per block, threads
 // over the row-major entries.

@@ -91,3 +91,25 @@ def test_bad_calls_raise():
     with pytest.raises(H2PHipError) as ei:
         m.matmul_dev(999, 2, 0, 1, 0, 1)
     assert ei.value.code == abi.EINVAL
+
+
+@pytest.mark.parametrize("name", ["coulomb", "coulomb_diag", "gaussian", "matern32"])
+def test_matvec_fp32mix_within_build_tolerance(name):
+    """FP32 kernel values and block products, FP64 U sweeps and outputs:
+    error against the FP64 oracle stays within the H2 build tolerance."""
+    h2, _ = problem(name)
+    x = _x(h2)
+    y_ref = O.OracleH2(h2, BD_JIT=1).matvec(x, n_thread=8)
+    m = H2MatHIP(h2, mode="jit", prec="fp32mix")
+    y = m.matvec(x)
+    err = rel(y, y_ref)
+    print(f"[{name} fp32mix] rel err vs FP64 oracle = {err:.3e}", flush=True)
+    assert err < h2.QR_stop_tol
+    assert np.array_equal(y, m.matvec(x))   # deterministic
+
+
+def test_fp32mix_refused_where_unavailable():
+    h2, _ = problem("tiny_rpy")
+    with pytest.raises(H2PHipError) as ei:
+        H2MatHIP(h2, mode="jit", prec="fp32mix")
+    assert ei.value.code == abi.ENOSYS
