@@ -83,9 +83,10 @@ struct h2p_hip_handle {
     int *d_fwd = nullptr;
     BlkTask *d_blk = nullptr;
     FwdTask *d_fwdt = nullptr;
-    BwdTask *d_bwdt = nullptr;
+    BwdTask2 *d_bwdt = nullptr;
     RedTask *d_red = nullptr;
-    LeafTask *d_leaf = nullptr;
+    LeafTask2 *d_leaf = nullptr;
+    double *d_yp = nullptr;      // S: push-down values of the downward sweep
     long long *d_contrib = nullptr;
     double *d_mm_x = nullptr, *d_mm_y = nullptr;  // matmul staging
     size_t mm_cap = 0;
@@ -107,7 +108,7 @@ static void free_handle(h2p_hip_handle *h) {
     if (!h) return;
     void *ptrs[] = {h->d_coord, h->d_U, h->d_UT, h->d_blob, h->d_xin, h->d_y1, h->d_arena,
                     h->d_fwd, h->d_blk, h->d_fwdt, h->d_bwdt, h->d_red, h->d_leaf,
-                    h->d_contrib, h->d_mm_x, h->d_mm_y};
+                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
@@ -159,7 +160,7 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
     if (prof) HIP_TRY(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
     for (const Level &L : h->fwd_levels)
-        hipLaunchKernelGGL(k_fwd, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
+        hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
     if (prof) HIP_TRY(hipEventRecord(ev[1], s));
     if (h->n_blk > 0) {
         if (h->mode == H2P_HIP_AOT)
@@ -172,10 +173,10 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
     if (prof) HIP_TRY(hipEventRecord(ev[3], s));
     for (const Level &L : h->bwd_levels)
-        hipLaunchKernelGGL(k_bwd, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1);
+        hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
     if (prof) HIP_TRY(hipEventRecord(ev[4], s));
-    hipLaunchKernelGGL(k_leaf_final, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
-                       h->d_UT, h->d_y1, h->d_fwd, d_y);
+    hipLaunchKernelGGL(k_leaf_final2, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
+                       h->d_UT, h->d_yp, h->d_fwd, d_y);
     HIP_TRY(hipGetLastError());
     if (prof) {
         HIP_TRY(hipEventRecord(ev[5], s));
@@ -412,11 +413,21 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         return (long long)a.nrow * a.ncol > (long long)b.nrow * b.ncol;
     });
     h->n_blk = (int)blk.size();
-    // ---- reduction into y1, and the contribution list
+    // ---- active nodes of the downward sweep (y1 "set", src/H2Pack_matvec.c:251-256, :802-838)
+    std::vector<char> active(nn, 0), has_push(nn, 0);
+    for (int lv = g->min_adm_level; lv <= g->max_level; lv++)
+        for (int j = 0; j < g->level_n_node[lv]; j++) {
+            int n = g->level_nodes[(size_t)lv * nlf + j];
+            const int p = parent[n];
+            has_push[n] = (p >= 0 && g->node_level[p] >= g->min_adm_level && active[p] && g->n_child[p] > 0) ? 1 : 0;
+            active[n] = ((g->node_n_r_adm[n] > 0 || has_push[n]) && g->U_ncol[n] > 0) ? 1 : 0;
+            if (!active[n]) has_push[n] = 0;
+        }
+    // ---- non-leaf skeleton sums (leaves sum their own in the output kernel)
     std::vector<long long> contrib;
     std::vector<RedTask> red;
     for (int n = 0; n < nn; n++) {
-        if (g->U_ncol[n] == 0) continue;
+        if (g->U_ncol[n] == 0 || g->n_child[n] == 0) continue;
         RedTask t{};
         t.dst = soff[n];
         t.len = g->U_ncol[n];
@@ -426,45 +437,47 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         red.push_back(t);
     }
     h->n_red = (int)red.size();
-    // ---- active nodes of the downward sweep (y1 "set", :251-256, :802-838)
-    std::vector<char> active(nn, 0);
-    for (int lv = g->min_adm_level; lv <= g->max_level; lv++)
-        for (int j = 0; j < g->level_n_node[lv]; j++) {
-            int n = g->level_nodes[(size_t)lv * nlf + j];
-            bool a = g->node_n_r_adm[n] > 0;
-            if (parent[n] >= 0 && g->node_level[parent[n]] >= g->min_adm_level && active[parent[n]]) a = true;
-            active[n] = (a && g->U_ncol[n] > 0) ? 1 : 0;
-        }
-    std::vector<BwdTask> bwdt;
+    // ---- downward sweep: one task per 256-row chunk, level by level
+    std::vector<BwdTask2> bwdt;
     for (int lv = g->min_adm_level; lv <= g->max_level; lv++) {
         Level L{(int)bwdt.size(), 0};
         for (int j = 0; j < g->level_n_node[lv]; j++) {
             int n = g->level_nodes[(size_t)lv * nlf + j];
             if (!active[n] || g->n_child[n] == 0) continue;
-            BwdTask t{};
-            t.ut_off = uoff[n];
-            t.nrow = g->U_nrow[n];
-            t.r = g->U_ncol[n];
-            t.in_off = soff[n];
-            t.out_off = soff[g->children[(size_t)n * mc]];
-            bwdt.push_back(t);
+            if (g->U_ncol[n] > H2P_MAXR) return bail(fail(H2P_HIP_ENOSYS, "rank above 2048"));
+            for (int r0 = 0; r0 < g->U_nrow[n]; r0 += 256) {
+                BwdTask2 t{};
+                t.ut_off = uoff[n];
+                t.nrow = g->U_nrow[n];
+                t.r = g->U_ncol[n];
+                t.y1_off = soff[n];
+                t.push = has_push[n];
+                t.out_off = soff[g->children[(size_t)n * mc]];
+                t.row0 = r0;
+                bwdt.push_back(t);
+            }
         }
         L.n = (int)bwdt.size() - L.off;
         if (L.n > 0) h->bwd_levels.push_back(L);
     }
-    std::vector<LeafTask> leaf;
+    std::vector<LeafTask2> leaf;
     for (int i = 0; i < nlf; i++) {
         int n = g->leaf_nodes[i];
-        LeafTask t{};
+        LeafTask2 t{};
         t.nrow = npt(n) * kd;
         t.r = active[n] ? g->U_ncol[n] : 0;
-        t.active = active[n];
+        if (t.r > H2P_MAXR) return bail(fail(H2P_HIP_ENOSYS, "rank above 2048"));
+        t.push = has_push[n];
         t.ut_off = uoff[n];
-        t.y1_off = soff[n];
+        t.s_off = soff[n];
         t.vec_s = g->mat_cluster[2 * n];
-        t.cbeg = (int)contrib.size();
+        t.cb1 = (int)contrib.size();
+        if (t.r > 0)
+            for (long long o : dest_y1[n]) contrib.push_back(o);
+        t.ce1 = (int)contrib.size();
+        t.cb0 = (int)contrib.size();
         for (long long o : dest_leaf[n]) contrib.push_back(o);
-        t.cend = (int)contrib.size();
+        t.ce0 = (int)contrib.size();
         leaf.push_back(t);
     }
     h->n_leaf = (int)leaf.size();
@@ -477,8 +490,9 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         for (const FwdTask &t : fwdt)
             if (!in(t.in_off, t.nrow, XIN) || !in(t.out_off, t.r, XIN) || !in(t.u_off, (long long)t.nrow * t.r, utot))
                 return bail(fail(H2P_HIP_EINVAL, "upward task out of range"));
-        for (const BwdTask &t : bwdt)
-            if (!in(t.in_off, t.r, Y1) || !in(t.out_off, t.nrow, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot))
+        for (const BwdTask2 &t : bwdt)
+            if (!in(t.y1_off, t.r, Y1) || !in(t.out_off, t.nrow, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot) ||
+                t.row0 < 0 || t.row0 >= t.nrow || t.r > H2P_MAXR)
                 return bail(fail(H2P_HIP_EINVAL, "downward task out of range"));
         for (const RedTask &t : red) {
             if (!in(t.dst, t.len, Y1) || t.cbeg < 0 || t.cend < t.cbeg || t.cend > NC)
@@ -486,12 +500,15 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
             for (int k = t.cbeg; k < t.cend; k++)
                 if (!in(contrib[k], t.len, AR)) return bail(fail(H2P_HIP_EINVAL, "reduction source out of range"));
         }
-        for (const LeafTask &t : leaf) {
-            if (!in(t.vec_s, t.nrow, N) || t.cbeg < 0 || t.cend < t.cbeg || t.cend > NC)
+        for (const LeafTask2 &t : leaf) {
+            if (!in(t.vec_s, t.nrow, N) || t.cb1 < 0 || t.ce1 < t.cb1 || t.cb0 != t.ce1 || t.ce0 < t.cb0 ||
+                t.ce0 > NC || t.r > H2P_MAXR)
                 return bail(fail(H2P_HIP_EINVAL, "leaf task out of range"));
-            if (t.active && (!in(t.y1_off, t.r, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot)))
+            if (t.r > 0 && (!in(t.s_off, t.r, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot)))
                 return bail(fail(H2P_HIP_EINVAL, "leaf task out of range"));
-            for (int k = t.cbeg; k < t.cend; k++)
+            for (int k = t.cb1; k < t.ce1; k++)
+                if (!in(contrib[k], t.r, AR)) return bail(fail(H2P_HIP_EINVAL, "leaf source out of range"));
+            for (int k = t.cb0; k < t.ce0; k++)
                 if (!in(contrib[k], t.nrow, AR)) return bail(fail(H2P_HIP_EINVAL, "leaf source out of range"));
         }
         for (const BlkTask &t : blk) {
@@ -520,6 +537,7 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     if ((rc = dev_upload(&h->d_contrib, contrib, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
+    if ((rc = dev_alloc(&h->d_yp, (size_t)S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_arena, (size_t)arena, acct))) return bail(rc);
     if (hipMemset(h->d_xin, 0, sizeof(double) * ((size_t)N + S)) != hipSuccess)
         return bail(fail(H2P_HIP_EDEVICE, "hipMemset failed"));

@@ -140,6 +140,139 @@ __global__ void __launch_bounds__(256) k_leaf_final(const LeafTask *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Sweep kernels, second form.
+//
+// Upward sweep: one workgroup per node; lanes own up to 256 columns of U at
+// once (4 strips), the four waves take contiguous quarters of the rows and
+// keep 4 rows x 4 strips of loads in flight; the four wave partials are
+// added in LDS in a fixed order.
+__global__ void __launch_bounds__(256) k_fwd2(const FwdTask *__restrict__ tasks,
+                                              const double *__restrict__ U,
+                                              double *__restrict__ xin) {
+    __shared__ double red[4][4][H2P_WAVE];
+    const FwdTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double *Ut = U + t.u_off;
+    const double *in = xin + t.in_off;
+    const int q = (t.nrow + 3) >> 2;
+    const int i0 = w * q, i1 = min(t.nrow, i0 + q);
+    for (int cs = 0; cs < t.r; cs += 4 * H2P_WAVE) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        int cc[4];
+        bool ok[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cc[k] = cs + k * H2P_WAVE + lane;
+            ok[k] = cc[k] < t.r;
+            cc[k] = ok[k] ? cc[k] : 0;
+        }
+        int i = i0;
+        for (; i + 3 < i1; i += 4) {
+            double u[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) u[a][k] = Ut[(long long)(i + a) * t.r + cc[k]];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const double v = in[i + a];
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc[k] += u[a][k] * v;
+            }
+        }
+        for (; i < i1; i++) {
+            const double v = in[i];
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc[k] += Ut[(long long)i * t.r + cc[k]] * v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++) red[k][w][lane] = acc[k];
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (ok[k])
+                    xin[t.out_off + cc[k]] = (red[k][0][lane] + red[k][1][lane]) + (red[k][2][lane] + red[k][3][lane]);
+        }
+    }
+}
+
+// Downward sweep, one workgroup per 256-row chunk of a node:
+// yp[child rows] = U (y1 + yp)[node], U^T stored row-major.  The push-down
+// buffer yp is written exactly once per row (each child has one parent).
+struct BwdTask2 {
+    long long ut_off;
+    int nrow, r, y1_off, push, out_off, row0;
+};
+#define H2P_MAXR 2048
+__global__ void __launch_bounds__(256) k_bwd2(const BwdTask2 *__restrict__ tasks,
+                                              const double *__restrict__ UT,
+                                              const double *__restrict__ y1,
+                                              double *__restrict__ yp) {
+    __shared__ double yv[H2P_MAXR];
+    const BwdTask2 t = tasks[blockIdx.x];
+    for (int c = threadIdx.x; c < t.r; c += blockDim.x)
+        yv[c] = y1[t.y1_off + c] + (t.push ? yp[t.y1_off + c] : 0.0);
+    __syncthreads();
+    const int row = t.row0 + threadIdx.x;
+    if (row >= t.nrow) return;
+    const double *ut = UT + t.ut_off + row;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = 0;
+    for (; c + 3 < t.r; c += 4) {
+        a0 += ut[(long long)c * t.nrow] * yv[c];
+        a1 += ut[(long long)(c + 1) * t.nrow] * yv[c + 1];
+        a2 += ut[(long long)(c + 2) * t.nrow] * yv[c + 2];
+        a3 += ut[(long long)(c + 3) * t.nrow] * yv[c + 3];
+    }
+    for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
+    yp[t.out_off + row] = (a0 + a1) + (a2 + a3);
+}
+
+// Leaf outputs, second form: the leaf's skeleton value is its own block
+// partials plus its parent's push-down; then U y1 plus the point partials
+// go straight to the caller's order.
+struct LeafTask2 {
+    long long ut_off;
+    int nrow, r, s_off, push, vec_s;
+    int cb1, ce1;   // contributions in skeleton space (length r)
+    int cb0, ce0;   // contributions in point space (length nrow)
+    int pad;
+};
+__global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict__ tasks,
+                                                     const long long *__restrict__ contrib,
+                                                     const double *__restrict__ arena,
+                                                     const double *__restrict__ UT,
+                                                     const double *__restrict__ yp,
+                                                     const int *__restrict__ fwd,
+                                                     double *__restrict__ y) {
+    __shared__ double yv[H2P_MAXR];
+    const LeafTask2 t = tasks[blockIdx.x];
+    for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
+        double v = t.push ? yp[t.s_off + c] : 0.0;
+        for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
+        yv[c] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) {
+        const double *ut = UT + t.ut_off + i;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int c = 0;
+        for (; c + 3 < t.r; c += 4) {
+            a0 += ut[(long long)c * t.nrow] * yv[c];
+            a1 += ut[(long long)(c + 1) * t.nrow] * yv[c + 1];
+            a2 += ut[(long long)(c + 2) * t.nrow] * yv[c + 2];
+            a3 += ut[(long long)(c + 3) * t.nrow] * yv[c + 3];
+        }
+        for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
+        double v = (a0 + a1) + (a2 + a3);
+        for (int k = t.cb0; k < t.ce0; k++) v += arena[contrib[k] + i];
+        y[fwd[t.vec_s + i]] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Grouped bi-GEMV over stored blocks (AOT), the products of
 // CBLAS_BI_GEMV (src/H2Pack_matvec.c:26-65) as used by the intermediate
 // (:303-381) and dense (:864-934) phases.  One workgroup per block: lanes
