@@ -75,26 +75,35 @@ __device__ __forceinline__ void rpy_coef(double dx, double dy, double dz,
 }
 
 // 2^x for x <= 0 in FP64: x = n + f with n = rint(x), f in [-1/2, 1/2]
-// (exact, the reduction is in base 2), 2^f = e^(f ln 2) by its Taylor series
-// through degree 12 (truncation below 2.5e-16 relative), then scaled by 2^n
-// with ldexp, which returns 0 once n drops below the subnormal range.
+// (exact, the reduction is in base 2), 2^f by a degree-10 polynomial
+// interpolating 2^f at the 11 Chebyshev nodes of [-1/2, 1/2] (coefficients
+// computed in extended precision; max relative error 4.5e-16 with Horner in
+// FP64), then scaled by 2^n with ldexp, which returns 0 once n drops below
+// the subnormal range.
 __device__ __forceinline__ double exp2_nonpos(double x) {
     const double n = __builtin_rint(x);
     const double f = x - n;
-    double p = 2.5678435993488206e-11;
-    p = __builtin_fma(p, f, 4.4455382718708116e-10);
-    p = __builtin_fma(p, f, 7.054911620801123e-09);
-    p = __builtin_fma(p, f, 1.01780860092397e-07);
-    p = __builtin_fma(p, f, 1.321548679014431e-06);
-    p = __builtin_fma(p, f, 1.5252733804059841e-05);
-    p = __builtin_fma(p, f, 0.0001540353039338161);
-    p = __builtin_fma(p, f, 0.0013333558146428443);
-    p = __builtin_fma(p, f, 0.009618129107628477);
-    p = __builtin_fma(p, f, 0.05550410866482158);
-    p = __builtin_fma(p, f, 0.24022650695910072);
-    p = __builtin_fma(p, f, 0.6931471805599453);
+    double p = 7.072585489423359e-09;
+    p = __builtin_fma(p, f, 1.0208690645080226e-07);
+    p = __builtin_fma(p, f, 1.3215442588981224e-06);
+    p = __builtin_fma(p, f, 1.525265725816688e-05);
+    p = __builtin_fma(p, f, 0.00015403530441732765);
+    p = __builtin_fma(p, f, 0.0013333558230168993);
+    p = __builtin_fma(p, f, 0.009618129107606902);
+    p = __builtin_fma(p, f, 0.05550410866444769);
+    p = __builtin_fma(p, f, 0.24022650695910097);
+    p = __builtin_fma(p, f, 0.69314718055995);
     p = __builtin_fma(p, f, 1.0);
     return __builtin_amdgcn_ldexp(p, (int)n);
+}
+
+// 1/sqrt(q) for q > 0 in FP64: the hardware estimate refined by one
+// Newton-Raphson step with its second-order term, e = 1 - q y^2,
+// y <- y + y e (1/2 + 3/8 e); the error of the estimate is cubed.
+__device__ __forceinline__ double rsqrt_refined(double q) {
+    const double y = __builtin_amdgcn_rsq(q);
+    const double e = __builtin_fma(-q * y, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
 }
 
 // Scalar kernels on prepared squared distances.  Coordinates are scaled
@@ -105,11 +114,16 @@ __device__ __forceinline__ double exp2_nonpos(double x) {
 template <int KID>
 __device__ __forceinline__ double kprep(double q, double diag) {
     if constexpr (KID == KC) {
-        return (q == 0.0) ? diag : rsqrt(q);
+        const bool z = q == 0.0;
+        const double y = rsqrt_refined(z ? 1.0 : q);   // branch-free
+        return z ? diag : y;
     } else if constexpr (KID == KG) {
         return exp2_nonpos(-q);
     } else {
-        const double s = __builtin_sqrt(q);
+        // sqrt(q) as q / sqrt(q); the clamp keeps q = 0 (and a rounded
+        // negative q from the norm-expanded distance) finite, s ~ 1e-150
+        const double qq = __builtin_fmax(q, 1e-300);
+        const double s = qq * rsqrt_refined(qq);
         return (1.0 + s) * exp2_nonpos(-1.4426950408889634 * s);
     }
 }
@@ -121,7 +135,8 @@ __device__ __forceinline__ float kprep(float q, float diag) {
     } else if constexpr (KID == KG) {
         return __builtin_amdgcn_exp2f(-q);
     } else {
-        const float s = __builtin_sqrtf(q);
+        const float qq = __builtin_fmaxf(q, 1e-30f);
+        const float s = qq * __builtin_amdgcn_rsqf(qq);
         return (1.0f + s) * __builtin_amdgcn_exp2f(-1.4426950408889634f * s);
     }
 }

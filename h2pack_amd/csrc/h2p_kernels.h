@@ -473,8 +473,19 @@ __global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ 
 //    pass as FP32 and are folded into the FP64 partial outputs; the column
 //    sums are FP32 over tiles of 16 rows, folded into FP64 registers.
 #define H2P_JIT_ROWS 256                 // target rows staged per chunk
+// Staged target row: prepared coordinates, squared norm (norm-expanded
+// distances), and the input value the transposed product multiplies.
 template <typename T>
-struct JitRow { T x, y, z, v; };
+struct JitRow { T x, y, z, n, v, pad; };
+template <>
+struct JitRow<float> { float x, y, z, v; };   // FP32 rows carry no norm
+
+// FP64 Gaussian / Matern form the squared distance as |r|^2 + |s|^2 - 2 r.s
+// (one add + three FMA instead of three subtractions, a multiply and two
+// FMA) from block-local coordinates; Coulomb keeps differences so that r = 0
+// is detected exactly, and FP32 keeps differences for accuracy.
+template <int KID, typename T>
+struct JitNorm { static constexpr bool value = sizeof(T) == 8 && (KID == KG || KID == KM); };
 
 template <int KID, typename T, int NS>
 __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
@@ -482,10 +493,11 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
                                           double scale, double diag, double ox, double oy, double oz,
                                           const double *__restrict__ xin, double *__restrict__ arena,
                                           JitRow<T> *rows, double (*red)[4][H2P_WAVE], bool first) {
+    constexpr bool NORM = JitNorm<KID, T>::value;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *cy = cx + cstride, *cz = cx + 2 * cstride;
     const T dg = (T)diag;
-    T sx[NS], sy[NS], sz[NS], x0[NS];
+    T sx[NS], sy[NS], sz[NS], sn[NS], x0[NS];
     double acc1d[NS];
     T acc1[NS];
 #pragma unroll
@@ -493,9 +505,18 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
         const int p = k * H2P_WAVE + lane;
         const bool ok = p < npts;
         const int g = t.c1 + pbase + (ok ? p : 0);
-        sx[k] = (T)((cx[g] - ox) * scale);
-        sy[k] = (T)((cy[g] - oy) * scale);
-        sz[k] = (T)((cz[g] - oz) * scale);
+        const double ux = (cx[g] - ox) * scale, uy = (cy[g] - oy) * scale, uz = (cz[g] - oz) * scale;
+        if constexpr (NORM) {
+            sx[k] = (T)(-2.0 * ux);
+            sy[k] = (T)(-2.0 * uy);
+            sz[k] = (T)(-2.0 * uz);
+            sn[k] = (T)(ux * ux + uy * uy + uz * uz);
+        } else {
+            sx[k] = (T)ux;
+            sy[k] = (T)uy;
+            sz[k] = (T)uz;
+            sn[k] = (T)0;
+        }
         x0[k] = ok ? (T)xin[t.in0 + pbase + p] : (T)0;
         acc1[k] = (T)0;
         acc1d[k] = 0.0;
@@ -508,10 +529,15 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
         for (int q = threadIdx.x; q < nr_pad; q += blockDim.x) {
             const bool ok = q < nr;
             const int g = t.c0 + rc + (ok ? q : 0);
+            const double ux = (cx[g] - ox) * scale, uy = (cy[g] - oy) * scale, uz = (cz[g] - oz) * scale;
             JitRow<T> r;
-            r.x = (T)((cx[g] - ox) * scale);
-            r.y = (T)((cy[g] - oy) * scale);
-            r.z = (T)((cz[g] - oz) * scale);
+            r.x = (T)ux;
+            r.y = (T)uy;
+            r.z = (T)uz;
+            if constexpr (NORM) {
+                r.n = (T)(ux * ux + uy * uy + uz * uz);
+                r.pad = (T)0;
+            }
             r.v = (ok && has1) ? (T)xin[t.in1 + rc + q] : (T)0;
             rows[q] = r;
         }
@@ -529,8 +555,16 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
             for (int k = 0; k < NS; k++) {
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
-                    const T dx = rr[a].x - sx[k], dy = rr[a].y - sy[k], dz = rr[a].z - sz[k];
-                    const T q = dx * dx + dy * dy + dz * dz;
+                    T q;
+                    if constexpr (NORM) {
+                        q = rr[a].n + sn[k];
+                        q = __builtin_fma(rr[a].x, sx[k], q);
+                        q = __builtin_fma(rr[a].y, sy[k], q);
+                        q = __builtin_fma(rr[a].z, sz[k], q);
+                    } else {
+                        const T dx = rr[a].x - sx[k], dy = rr[a].y - sy[k], dz = rr[a].z - sz[k];
+                        q = dx * dx + dy * dy + dz * dz;
+                    }
                     const T kv = kprep<KID>(q, dg);
                     p[a] += kv * x0[k];
                     acc1[k] += kv * rr[a].v;
@@ -579,7 +613,7 @@ __global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__
     __shared__ double red[H2P_JIT_MAXNS][4][H2P_WAVE];
     const BlkTask t = tasks[blockIdx.x];
     double ox = 0.0, oy = 0.0, oz = 0.0;
-    if constexpr (sizeof(T) == 4) {   // block-local origin for FP32 coordinates
+    if constexpr (sizeof(T) == 4 || JitNorm<KID, T>::value) {   // block-local origin
         ox = cx[t.c0];
         oy = cx[cstride + t.c0];
         oz = cx[2 * cstride + t.c0];
