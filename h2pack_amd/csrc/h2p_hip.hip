@@ -22,6 +22,7 @@
 
 #include "h2p_hip.h"
 #include "h2p_kernels.h"
+#include "h2p_mm.h"
 
 using namespace h2p;
 
@@ -76,7 +77,7 @@ struct h2p_hip_handle {
     // device arrays
     double *d_coord = nullptr;   // xpt_dim x CT
     double *d_U = nullptr, *d_UT = nullptr;
-    double *d_blob = nullptr;    // AOT B then D
+    void *d_blob = nullptr;      // AOT B then D (double, or float in FP32-mixed mode)
     double *d_xin = nullptr;     // N + S
     double *d_y1 = nullptr;      // S
     double *d_arena = nullptr;
@@ -88,8 +89,15 @@ struct h2p_hip_handle {
     LeafTask2 *d_leaf = nullptr;
     double *d_yp = nullptr;      // S: push-down values of the downward sweep
     long long *d_contrib = nullptr;
-    double *d_mm_x = nullptr, *d_mm_y = nullptr;  // matmul staging
+    double *d_mm_x = nullptr, *d_mm_y = nullptr;  // staging of the host-pointer calls
     size_t mm_cap = 0;
+    // matmul (pull-form MFMA tasks, built at upload)
+    MMTask *d_mmt = nullptr;
+    MMSrc *d_mms = nullptr;
+    std::vector<Level> mm_fwd_levels, mm_bwd_levels;
+    Level mm_intmd{0, 0}, mm_leaf{0, 0};
+    double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, H2P_MM_MAXNVP wide
+    bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
     long long arena_len = 0;
@@ -108,7 +116,7 @@ static void free_handle(h2p_hip_handle *h) {
     if (!h) return;
     void *ptrs[] = {h->d_coord, h->d_U, h->d_UT, h->d_blob, h->d_xin, h->d_y1, h->d_arena,
                     h->d_fwd, h->d_blk, h->d_fwdt, h->d_bwdt, h->d_red, h->d_leaf,
-                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp};
+                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp, h->d_mmt, h->d_mms, h->d_panels};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
@@ -124,8 +132,12 @@ static void launch_jit(h2p_hip_handle *h, hipStream_t s) {
 
 template <int KID, int KD>
 static void launch_build(h2p_hip_handle *h, hipStream_t s) {
-    hipLaunchKernelGGL((k_build_blocks<KID, KD>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
-                       h->d_coord, h->CT, h->kp, h->d_blob);
+    if (h->prec == H2P_HIP_FP32MIX)
+        hipLaunchKernelGGL((k_build_blocks<KID, KD, float>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                           h->d_coord, h->CT, h->kp, (float *)h->d_blob);
+    else
+        hipLaunchKernelGGL((k_build_blocks<KID, KD, double>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                           h->d_coord, h->CT, h->kp, (double *)h->d_blob);
 }
 
 template <int KID, typename T>
@@ -163,8 +175,12 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
         hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
     if (prof) HIP_TRY(hipEventRecord(ev[1], s));
     if (h->n_blk > 0) {
-        if (h->mode == H2P_HIP_AOT)
-            hipLaunchKernelGGL(k_blocks_aot, dim3(h->n_blk), dim3(256), 0, s, h->d_blk, h->d_blob, h->d_xin, h->d_arena);
+        if (h->mode == H2P_HIP_AOT && h->prec == H2P_HIP_FP32MIX)
+            hipLaunchKernelGGL(k_blocks_aot<float>, dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                               (const float *)h->d_blob, h->d_xin, h->d_arena);
+        else if (h->mode == H2P_HIP_AOT)
+            hipLaunchKernelGGL(k_blocks_aot<double>, dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                               (const double *)h->d_blob, h->d_xin, h->d_arena);
         else
             dispatch_jit(h, s);
     }
@@ -202,6 +218,22 @@ __global__ void k_transpose(const double *__restrict__ in, long long ldi, double
     }
 }
 
+template <int KID, int KD>
+static void launch_mm(h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
+    if (L.n <= 0) return;
+    hipLaunchKernelGGL((k_mm<KID, KD>), dim3(L.n), dim3(256), 0, s, h->d_mmt + L.off, h->d_mms,
+                       h->d_U, h->d_UT, (const double *)h->d_blob, h->d_coord, h->CT, h->kp, P);
+}
+
+static void dispatch_mm(h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
+    switch (h->kid) {
+        case KC: launch_mm<KC, 1>(h, L, P, s); break;
+        case KG: launch_mm<KG, 1>(h, L, P, s); break;
+        case KM: launch_mm<KM, 1>(h, L, P, s); break;
+        default: launch_mm<KR, 3>(h, L, P, s); break;
+    }
+}
+
 extern "C" {
 
 const char *h2p_hip_last_error(void) { return g_err.c_str(); }
@@ -213,8 +245,8 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     if (!g || !krnl) return fail(H2P_HIP_EINVAL, "NULL representation or kernel");
     if (mode != H2P_HIP_AOT && mode != H2P_HIP_JIT) return fail(H2P_HIP_EINVAL, "bad mode");
     if (prec != H2P_HIP_FP64 && prec != H2P_HIP_FP32MIX) return fail(H2P_HIP_EINVAL, "bad precision");
-    if (prec == H2P_HIP_FP32MIX && (mode != H2P_HIP_JIT || krnl->id == KR))
-        return fail(H2P_HIP_ENOSYS, "FP32-mixed is available for JIT mode with the scalar kernels");
+    if (prec == H2P_HIP_FP32MIX && mode == H2P_HIP_JIT && krnl->id == KR)
+        return fail(H2P_HIP_ENOSYS, "FP32-mixed JIT is available for the scalar kernels");
     if (krnl->id < 0 || krnl->id > 3) return fail(H2P_HIP_EINVAL, "bad kernel id");
     const int kd = g->krnl_dim;
     if (g->pt_dim != 3) return fail(H2P_HIP_EINVAL, "pt_dim must be 3");
@@ -481,6 +513,128 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         leaf.push_back(t);
     }
     h->n_leaf = (int)leaf.size();
+    // ---- matmul tasks (pull form, one task per destination)
+    std::vector<MMTask> mmt;
+    std::vector<MMSrc> mms;
+    const bool jit = mode == H2P_HIP_JIT;
+    {
+        // per-destination source lists, in pair order
+        std::vector<std::vector<MMSrc>> y1_src(nn), leaf_src(nn);
+        auto set_start = [&](int n, bool pts) { return pts ? g->pt_cluster[2 * n] : g->n_point + joff[n]; };
+        for (int i = 0; i < g->n_r_adm_pair; i++) {
+            const int n0 = g->r_adm_pairs[2 * i], n1 = g->r_adm_pairs[2 * i + 1];
+            const int l0 = g->node_level[n0], l1 = g->node_level[n1];
+            const int br = g->B_nrow[i], bc = g->B_ncol[i];
+            if (br == 0 || bc == 0) continue;
+            const bool p0 = l0 < l1, p1 = l0 > l1;      // side given by points
+            MMSrc a{}, b{};
+            a.kind = b.kind = jit ? MM_A_JIT : MM_A_BD;
+            a.a_off = b.a_off = (long long)g->B_ptr[i];
+            a.lda = b.lda = bc;
+            // entry of n0: op = B
+            a.trans = 0; a.K = bc;
+            a.c0 = set_start(n0, p0); a.c1 = set_start(n1, p1);
+            a.x_space = p1 ? MM_PX : MM_Y0; a.x_off = p1 ? g->mat_cluster[2 * n1] : soff[n1];
+            (p0 ? leaf_src[n0] : y1_src[n0]).push_back(a);
+            // entry of n1: op = B^T
+            b.trans = 1; b.K = br;
+            b.c0 = set_start(n1, p1); b.c1 = set_start(n0, p0);
+            b.x_space = p0 ? MM_PX : MM_Y0; b.x_off = p0 ? g->mat_cluster[2 * n0] : soff[n0];
+            (p1 ? leaf_src[n1] : y1_src[n1]).push_back(b);
+        }
+        for (int i = 0; i < nD; i++) {
+            int n0, n1;
+            if (i < nlf) n0 = n1 = g->leaf_nodes[i];
+            else { n0 = g->r_inadm_pairs[2 * (i - nlf)]; n1 = g->r_inadm_pairs[2 * (i - nlf) + 1]; }
+            const int dr = g->D_nrow[i], dc = g->D_ncol[i];
+            if (dr == 0 || dc == 0) continue;
+            MMSrc a{};
+            a.kind = jit ? MM_A_JIT : MM_A_BD;
+            a.a_off = (long long)(btot + g->D_ptr[i]);
+            a.lda = dc;
+            a.trans = 0; a.K = dc;
+            a.c0 = g->pt_cluster[2 * n0]; a.c1 = g->pt_cluster[2 * n1];
+            a.x_space = MM_PX; a.x_off = g->mat_cluster[2 * n1];
+            leaf_src[n0].push_back(a);
+            if (i >= nlf) {
+                MMSrc b = a;
+                b.trans = 1; b.K = dr;
+                b.c0 = g->pt_cluster[2 * n1]; b.c1 = g->pt_cluster[2 * n0];
+                b.x_off = g->mat_cluster[2 * n0];
+                leaf_src[n1].push_back(b);
+            }
+        }
+        auto add_task = [&](int space, int off, int M, int accum, const std::vector<MMSrc> &src) {
+            MMTask t{};
+            t.dst_space = space; t.dst_off = off; t.M = M; t.accumulate = accum;
+            t.sbeg = (int)mms.size();
+            for (const MMSrc &x : src) mms.push_back(x);
+            t.send = (int)mms.size();
+            mmt.push_back(t);
+        };
+        auto work = [&](const MMTask &t) {
+            long long wsum = 0;
+            for (int k = t.sbeg; k < t.send; k++) wsum += (long long)t.M * mms[k].K;
+            return wsum;
+        };
+        // upward sweep, level by level
+        for (int lv = g->max_level; lv >= g->min_adm_level; lv--) {
+            Level L{(int)mmt.size(), 0};
+            for (int j = 0; j < g->level_n_node[lv]; j++) {
+                int n = g->level_nodes[(size_t)lv * nlf + j];
+                if (g->U_ncol[n] == 0) continue;
+                MMSrc a{};
+                a.kind = MM_A_UT; a.a_off = uoff[n]; a.lda = g->U_nrow[n]; a.trans = 0; a.K = g->U_nrow[n];
+                if (g->n_child[n] == 0) { a.x_space = MM_PX; a.x_off = g->mat_cluster[2 * n]; }
+                else { a.x_space = MM_Y0; a.x_off = soff[g->children[(size_t)n * mc]]; }
+                add_task(MM_Y0, soff[n], g->U_ncol[n], 0, std::vector<MMSrc>{a});
+            }
+            L.n = (int)mmt.size() - L.off;
+            if (L.n > 0) h->mm_fwd_levels.push_back(L);
+        }
+        // coupling into Y1 (every node with a basis, so Y1 is always written)
+        {
+            Level L{(int)mmt.size(), 0};
+            for (int n = 0; n < nn; n++)
+                if (g->U_ncol[n] > 0) add_task(MM_Y1, soff[n], g->U_ncol[n], 0, y1_src[n]);
+            L.n = (int)mmt.size() - L.off;
+            std::stable_sort(mmt.begin() + L.off, mmt.end(), [&](const MMTask &x, const MMTask &y) { return work(x) > work(y); });
+            h->mm_intmd = L;
+        }
+        // downward sweep, level by level
+        for (int lv = g->min_adm_level; lv <= g->max_level; lv++) {
+            Level L{(int)mmt.size(), 0};
+            for (int j = 0; j < g->level_n_node[lv]; j++) {
+                int n = g->level_nodes[(size_t)lv * nlf + j];
+                if (!active[n] || g->n_child[n] == 0) continue;
+                MMSrc a{};
+                a.kind = MM_A_U; a.a_off = uoff[n]; a.lda = g->U_ncol[n]; a.trans = 0; a.K = g->U_ncol[n];
+                a.x_space = MM_Y1; a.x_off = soff[n];
+                add_task(MM_Y1, soff[g->children[(size_t)n * mc]], g->U_nrow[n], 1, std::vector<MMSrc>{a});
+            }
+            L.n = (int)mmt.size() - L.off;
+            if (L.n > 0) h->mm_bwd_levels.push_back(L);
+        }
+        // leaf outputs
+        {
+            Level L{(int)mmt.size(), 0};
+            for (int i = 0; i < nlf; i++) {
+                int n = g->leaf_nodes[i];
+                std::vector<MMSrc> src;
+                if (active[n]) {
+                    MMSrc a{};
+                    a.kind = MM_A_U; a.a_off = uoff[n]; a.lda = g->U_ncol[n]; a.trans = 0; a.K = g->U_ncol[n];
+                    a.x_space = MM_Y1; a.x_off = soff[n];
+                    src.push_back(a);
+                }
+                for (const MMSrc &x : leaf_src[n]) src.push_back(x);
+                add_task(MM_PY, g->mat_cluster[2 * n], npt(n) * kd, 0, src);
+            }
+            L.n = (int)mmt.size() - L.off;
+            std::stable_sort(mmt.begin() + L.off, mmt.end(), [&](const MMTask &x, const MMTask &y) { return work(x) > work(y); });
+            h->mm_leaf = L;
+        }
+    }
     // ---- every access of every task stays inside its buffer
     {
         const long long XIN = (long long)N + S, Y1 = S, AR = arena, NC = (long long)contrib.size();
@@ -520,6 +674,25 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         }
         for (int i = 0; i < N; i++)
             if (fwdidx_ok(g->fwd_pmt_idx[i], N) == false) return bail(fail(H2P_HIP_EINVAL, "fwd_pmt_idx out of range"));
+        const long long rows_of[4] = {N, S, S, N};
+        for (const MMTask &t : mmt) {
+            bool ok = t.dst_space >= 0 && t.dst_space < 4 && in(t.dst_off, t.M, rows_of[t.dst_space]) &&
+                      t.sbeg >= 0 && t.send >= t.sbeg && t.send <= (int)mms.size();
+            for (int k = t.sbeg; ok && k < t.send; k++) {
+                const MMSrc &x = mms[k];
+                ok = x.x_space >= 0 && x.x_space < 4 && in(x.x_off, x.K, rows_of[x.x_space]) && x.K >= 0;
+                if (!ok) break;
+                if (x.kind == MM_A_JIT) {
+                    ok = (t.M % kd == 0) && (x.K % kd == 0) && in(x.c0, t.M / kd, CT) && in(x.c1, x.K / kd, CT);
+                } else {
+                    const long long cap = (x.kind == MM_A_BD) ? BLOB : utot;
+                    const long long rows = x.trans ? x.K : t.M, cols = x.trans ? t.M : x.K;
+                    ok = (x.kind == MM_A_U || x.kind == MM_A_UT || x.kind == MM_A_BD) && x.lda >= cols &&
+                         (rows == 0 || in(x.a_off, (rows - 1) * (long long)x.lda + cols, cap));
+                }
+            }
+            if (!ok) return bail(fail(H2P_HIP_EINVAL, "matmul task out of range"));
+        }
     }
     // ---- device memory
     size_t *acct = &h->device_bytes;
@@ -535,6 +708,9 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     if ((rc = dev_upload(&h->d_red, red, acct))) return bail(rc);
     if ((rc = dev_upload(&h->d_leaf, leaf, acct))) return bail(rc);
     if ((rc = dev_upload(&h->d_contrib, contrib, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_mmt, mmt, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_mms, mms, acct))) return bail(rc);
+    h->mm_ok = !(mode == H2P_HIP_AOT && prec == H2P_HIP_FP32MIX);
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_yp, (size_t)S, acct))) return bail(rc);
@@ -542,12 +718,41 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     if (hipMemset(h->d_xin, 0, sizeof(double) * ((size_t)N + S)) != hipSuccess)
         return bail(fail(H2P_HIP_EDEVICE, "hipMemset failed"));
     if (mode == H2P_HIP_AOT) {
-        if ((rc = dev_alloc(&h->d_blob, btot + dtot, acct))) return bail(rc);
+        const bool f32 = prec == H2P_HIP_FP32MIX;
+        const size_t esz = f32 ? sizeof(float) : sizeof(double);
+        {
+            void *p = nullptr;
+            size_t bytes = esz * std::max<size_t>(btot + dtot, 1);
+            if (hipMalloc(&p, bytes) != hipSuccess) return bail(fail(H2P_HIP_ENOMEM, "hipMalloc of the AOT blocks failed"));
+            h->d_blob = p;
+            *acct += bytes;
+        }
         if (g->B_data && g->D_data) {
-            if (btot && hipMemcpy(h->d_blob, g->B_data, sizeof(double) * btot, hipMemcpyHostToDevice) != hipSuccess)
-                return bail(fail(H2P_HIP_EDEVICE, "B upload failed"));
-            if (dtot && hipMemcpy(h->d_blob + btot, g->D_data, sizeof(double) * dtot, hipMemcpyHostToDevice) != hipSuccess)
-                return bail(fail(H2P_HIP_EDEVICE, "D upload failed"));
+            if (!f32) {
+                double *blob = (double *)h->d_blob;
+                if (btot && hipMemcpy(blob, g->B_data, sizeof(double) * btot, hipMemcpyHostToDevice) != hipSuccess)
+                    return bail(fail(H2P_HIP_EDEVICE, "B upload failed"));
+                if (dtot && hipMemcpy(blob + btot, g->D_data, sizeof(double) * dtot, hipMemcpyHostToDevice) != hipSuccess)
+                    return bail(fail(H2P_HIP_EDEVICE, "D upload failed"));
+            } else {
+                // stage FP64 host blocks through a device buffer and round them to FP32
+                const size_t chunk = (size_t)32 << 20;
+                double *stage = nullptr;
+                if (hipMalloc(&stage, sizeof(double) * chunk) != hipSuccess) return bail(fail(H2P_HIP_ENOMEM, "hipMalloc failed"));
+                float *blob = (float *)h->d_blob;
+                const double *srcs[2] = {g->B_data, g->D_data};
+                const size_t lens[2] = {btot, dtot}, offs[2] = {0, btot};
+                int err = 0;
+                for (int part = 0; part < 2 && !err; part++)
+                    for (size_t o = 0; o < lens[part] && !err; o += chunk) {
+                        const size_t n = std::min(chunk, lens[part] - o);
+                        if (hipMemcpy(stage, srcs[part] + o, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) err = 1;
+                        else hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, 0, stage, blob + offs[part] + o, (long long)n);
+                    }
+                if (!err && hipDeviceSynchronize() != hipSuccess) err = 1;
+                (void)hipFree(stage);
+                if (err) return bail(fail(H2P_HIP_EDEVICE, "block upload failed"));
+            }
         } else if (!g->B_data && !g->D_data) {
             if (h->n_blk > 0) dispatch_build(h, 0);
             if (hipDeviceSynchronize() != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "device block build failed"));
@@ -566,7 +771,8 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     for (int i = 0; i < nn; i++) vec += 2 * (size_t)(g->U_nrow[i] + g->U_ncol[i]);
     for (int i = 0; i < g->n_r_adm_pair; i++) vec += 2 * (size_t)(g->B_nrow[i] + g->B_ncol[i]);
     for (int i = 0; i < nD; i++) vec += (size_t)(i < nlf ? 1 : 2) * (g->D_nrow[i] + g->D_ncol[i]);
-    I.hbm_bytes_model = 8 * (2 * (size_t)utot + (mode == H2P_HIP_AOT ? btot + dtot : 0) + vec);
+    const size_t blk_esz = (prec == H2P_HIP_FP32MIX) ? sizeof(float) : sizeof(double);
+    I.hbm_bytes_model = 8 * (2 * (size_t)utot + vec) + (mode == H2P_HIP_AOT ? blk_esz * (btot + dtot) : 0);
     I.B_pairs = B_pairs;
     I.D_pairs = D_pairs;
     static const int flops[4] = {14, 14, 17, 82};
@@ -609,37 +815,42 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
     if (!h || (n_vec > 0 && (!d_X || !d_Y))) return fail(H2P_HIP_EINVAL, "NULL argument");
     if (n_vec < 0) return fail(H2P_HIP_EINVAL, "n_vec < 0");
     if (n_vec == 0) return 0;
-    hipStream_t s = (hipStream_t)stream;
+    const bool rm = layout == H2P_HIP_ROW_MAJOR;
+    if (!rm && layout != H2P_HIP_COL_MAJOR) return fail(H2P_HIP_EINVAL, "bad layout");
     const int N = h->N;
-    if (layout == H2P_HIP_COL_MAJOR) {
-        if (ldx < N || ldy < N) return fail(H2P_HIP_EINVAL, "ld < krnl_mat_size");
-        for (int v = 0; v < n_vec; v++) {
-            int rc = run_matvec(h, d_X + (size_t)v * ldx, d_Y + (size_t)v * ldy, s);
-            if (rc) return rc;
-        }
-        return 0;
+    if (rm ? (ldx < n_vec || ldy < n_vec) : (ldx < N || ldy < N)) return fail(H2P_HIP_EINVAL, "bad leading dimension");
+    if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t rows_total = 2 * (size_t)N + 2 * (size_t)h->S;
+    if (!h->d_panels) {
+        if (hipMalloc(&h->d_panels, sizeof(double) * rows_total * H2P_MM_MAXNVP) != hipSuccess)
+            return fail(H2P_HIP_ENOMEM, "hipMalloc of the matmul panels failed");
+        h->device_bytes += sizeof(double) * rows_total * H2P_MM_MAXNVP;
+        h->info.device_bytes = h->device_bytes;
     }
-    if (layout != H2P_HIP_ROW_MAJOR) return fail(H2P_HIP_EINVAL, "bad layout");
-    if (ldx < n_vec || ldy < n_vec) return fail(H2P_HIP_EINVAL, "ld < n_vec");
-    // stage row-major panels as columns
-    if (h->mm_cap < (size_t)n_vec) {
-        if (h->d_mm_x) { (void)hipFree(h->d_mm_x); h->d_mm_x = nullptr; }
-        if (h->d_mm_y) { (void)hipFree(h->d_mm_y); h->d_mm_y = nullptr; }
-        size_t dummy = 0;
-        int rc;
-        if ((rc = dev_alloc(&h->d_mm_x, (size_t)N * n_vec, &dummy))) return rc;
-        if ((rc = dev_alloc(&h->d_mm_y, (size_t)N * n_vec, &dummy))) return rc;
-        h->mm_cap = n_vec;
+    // columns in chunks of at most 64 (the reference chunks by mm_max_n_vec,
+    // src/H2Pack_matmul.c:497-580)
+    for (int v0 = 0; v0 < n_vec; v0 += H2P_MM_MAXNVP) {
+        const int nv = std::min(H2P_MM_MAXNVP, n_vec - v0);
+        const int nvp = (nv + 15) & ~15;
+        MMPanels P;
+        P.nvp = nvp;
+        P.p[MM_PX] = h->d_panels;
+        P.p[MM_Y0] = P.p[MM_PX] + (size_t)N * nvp;
+        P.p[MM_Y1] = P.p[MM_Y0] + (size_t)h->S * nvp;
+        P.p[MM_PY] = P.p[MM_Y1] + (size_t)h->S * nvp;
+        const long long ne = (long long)N * nvp;
+        hipLaunchKernelGGL(k_mm_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, d_X, (long long)ldx,
+                           rm ? 1 : 0, v0, nv, h->d_fwd, N, P.p[MM_PX], nvp);
+        for (const Level &L : h->mm_fwd_levels) dispatch_mm(h, L, P, s);
+        dispatch_mm(h, h->mm_intmd, P, s);
+        for (const Level &L : h->mm_bwd_levels) dispatch_mm(h, L, P, s);
+        dispatch_mm(h, h->mm_leaf, P, s);
+        const long long no = (long long)N * nv;
+        hipLaunchKernelGGL(k_mm_scatter, dim3((unsigned)((no + 255) / 256)), dim3(256), 0, s, P.p[MM_PY], nvp,
+                           h->d_fwd, N, d_Y, (long long)ldy, rm ? 1 : 0, v0, nv);
+        HIP_TRY(hipGetLastError());
     }
-    dim3 g1((n_vec + 31) / 32, (N + 31) / 32), b(32, 8);
-    hipLaunchKernelGGL(k_transpose, g1, b, 0, s, d_X, (long long)ldx, h->d_mm_x, (long long)N, N, n_vec);
-    for (int v = 0; v < n_vec; v++) {
-        int rc = run_matvec(h, h->d_mm_x + (size_t)v * N, h->d_mm_y + (size_t)v * N, s);
-        if (rc) return rc;
-    }
-    dim3 g2((N + 31) / 32, (n_vec + 31) / 32);
-    hipLaunchKernelGGL(k_transpose, g2, b, 0, s, h->d_mm_y, (long long)N, d_Y, (long long)ldy, n_vec, N);
-    HIP_TRY(hipGetLastError());
     return 0;
 }
 

@@ -279,72 +279,110 @@ __global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict
 // own columns (NS strips of 64), the four waves take every fourth row; the
 // row sums are wave reductions, the column sums are reduced across waves in
 // LDS in a fixed order.
-template <int NS>
+template <typename TS, int NS>
 __device__ __forceinline__ void blk_aot_pass(const BlkTask &t, int cbase, int ncols,
-                                             const double *__restrict__ A,
+                                             const TS *__restrict__ A,
                                              const double *__restrict__ xin,
                                              double *__restrict__ arena,
                                              double (*red)[4][H2P_WAVE], bool first) {
+    // TS = double: FP64 products.  TS = float (FP32-mixed): blocks stored and
+    // multiplied in FP32; row sums leave the pass as FP32 into FP64 partials,
+    // column sums are FP32 over 16 rows and folded into FP64 registers.
+    // Each wave takes four consecutive rows per iteration (4 x NS loads in
+    // flight per lane) and reduces the four row sums with one butterfly.
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double x0[NS], acc1[NS];
+    TS x0[NS], acc1[NS];
+    double acc1d[NS];
+    int cc[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) {
-        int c = k * H2P_WAVE + lane;
-        x0[k] = (c < ncols) ? xin[t.in0 + cbase + c] : 0.0;
-        acc1[k] = 0.0;
+        const int c = k * H2P_WAVE + lane;
+        const bool ok = c < ncols;
+        cc[k] = ok ? c : 0;
+        x0[k] = ok ? (TS)xin[t.in0 + cbase + c] : (TS)0;
+        acc1[k] = (TS)0;
+        acc1d[k] = 0.0;
     }
     const double *in1 = xin + t.in1;
     const bool has1 = t.flags & 1;
-    for (int i = w; i < t.nrow; i += 4) {
-        const double *row = A + (long long)i * t.ncol + cbase;
-        const double v1 = has1 ? in1[i] : 0.0;
-        double p = 0.0;
+    int tile = 0;
+    for (int r0 = 4 * w; r0 < t.nrow; r0 += 16) {
+        TS p[4], v1[4], a[4][NS];
 #pragma unroll
-        for (int k = 0; k < NS; k++) {
-            int c = k * H2P_WAVE + lane;
-            double a = (c < ncols) ? row[c] : 0.0;
-            p += a * x0[k];
-            acc1[k] += a * v1;
+        for (int q = 0; q < 4; q++) {
+            const int i = min(r0 + q, t.nrow - 1);     // rows past the end repeat the last row
+            const bool ok = r0 + q < t.nrow;
+            const TS *row = A + (long long)i * t.ncol + cbase;
+#pragma unroll
+            for (int k = 0; k < NS; k++) a[q][k] = row[cc[k]];
+            v1[q] = (has1 && ok) ? (TS)in1[i] : (TS)0;
+            p[q] = (TS)0;
         }
-        p = wave_sum(p);
-        if (lane == 0) {
-            if (first) arena[t.out0 + i] = p;
-            else arena[t.out0 + i] += p;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                p[q] += a[q][k] * x0[k];
+                acc1[k] += a[q][k] * v1[q];
+            }
+        const TS s = wave_sum4<TS>(p[0], p[1], p[2], p[3]);
+        const int myrow = r0 + ((lane >> 4) & 3);
+        if ((lane & 15) == 0 && myrow < t.nrow) {
+            double *o = arena + t.out0 + myrow;
+            if (first) *o = (double)s;
+            else *o += (double)s;
+        }
+        if constexpr (sizeof(TS) == 4) {
+            if (++tile == 4) {
+                tile = 0;
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    acc1d[k] += (double)acc1[k];
+                    acc1[k] = (TS)0;
+                }
+            }
         }
     }
+#pragma unroll
+    for (int k = 0; k < NS; k++) acc1d[k] += (double)acc1[k];
     if (has1) {
 #pragma unroll
-        for (int k = 0; k < NS; k++) {
-            red[k][w][lane] = acc1[k];
-        }
+        for (int k = 0; k < NS; k++) red[k][w][lane] = acc1d[k];
         __syncthreads();
         for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
-            int k = c >> 6, l = c & 63;
+            const int k = c >> 6, l = c & 63;
             arena[t.out1 + cbase + c] = (red[k][0][l] + red[k][1][l]) + (red[k][2][l] + red[k][3][l]);
         }
         __syncthreads();
     }
 }
 
-#define H2P_AOT_MAXNS 8
+#define H2P_AOT_MAXNS 4
+template <typename TS>
 __global__ void __launch_bounds__(256) k_blocks_aot(const BlkTask *__restrict__ tasks,
-                                                    const double *__restrict__ blob,
+                                                    const TS *__restrict__ blob,
                                                     const double *__restrict__ xin,
                                                     double *__restrict__ arena) {
     __shared__ double red[H2P_AOT_MAXNS][4][H2P_WAVE];
     const BlkTask t = tasks[blockIdx.x];
-    const double *A = blob + t.data_off;
+    const TS *A = blob + t.data_off;
     for (int cb = 0; cb < t.ncol; cb += H2P_AOT_MAXNS * H2P_WAVE) {
         const int nc = min(t.ncol - cb, H2P_AOT_MAXNS * H2P_WAVE);
         const int ns = (nc + 63) >> 6;
         const bool first = (cb == 0);
-        if (ns <= 1) blk_aot_pass<1>(t, cb, nc, A, xin, arena, red, first);
-        else if (ns <= 2) blk_aot_pass<2>(t, cb, nc, A, xin, arena, red, first);
-        else if (ns <= 4) blk_aot_pass<4>(t, cb, nc, A, xin, arena, red, first);
-        else blk_aot_pass<8>(t, cb, nc, A, xin, arena, red, first);
+        if (ns <= 1) blk_aot_pass<TS, 1>(t, cb, nc, A, xin, arena, red, first);
+        else if (ns <= 2) blk_aot_pass<TS, 2>(t, cb, nc, A, xin, arena, red, first);
+        else if (ns <= 3) blk_aot_pass<TS, 3>(t, cb, nc, A, xin, arena, red, first);
+        else blk_aot_pass<TS, 4>(t, cb, nc, A, xin, arena, red, first);
         // the next column pass adds into out0 rows written by the same lanes
         __syncthreads();
     }
+}
+
+__global__ void k_f64_to_f32(const double *__restrict__ in, float *__restrict__ out, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)in[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -634,21 +672,22 @@ __global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__
 // Device-side evaluation of AOT blocks (H2P_build_B_AOT / H2P_build_D_AOT,
 // src/H2Pack_build.c:789-889, 1034-1140): one workgroup per block, threads
 // over the row-major entries.
-template <int KID, int KD>
+template <int KID, int KD, typename TS>
 __global__ void __launch_bounds__(256) k_build_blocks(const BlkTask *__restrict__ tasks,
                                                       const double *__restrict__ cx,
                                                       long long cstride, KParam kp,
-                                                      double *__restrict__ blob) {
+                                                      TS *__restrict__ blob) {
+    // values are evaluated in FP64 and stored as TS
     const BlkTask t = tasks[blockIdx.x];
     const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
-    double *A = blob + t.data_off;
+    TS *A = blob + t.data_off;
     const long long total = (long long)t.n0 * t.n1;
     for (long long e = threadIdx.x; e < total; e += blockDim.x) {
         int i = (int)(e / t.n1), j = (int)(e - (long long)i * t.n1);
         int gi = t.c0 + i, gj = t.c1 + j;
         double dx = cx[gi] - cx[gj], dy = cy[gi] - cy[gj], dz = cz[gi] - cz[gj];
         if constexpr (KD == 1) {
-            A[e] = kval<KID>(dx * dx + dy * dy + dz * dz, kp);
+            A[e] = (TS)kval<KID>(dx * dx + dy * dy + dz * dz, kp);
         } else {
             double t1, t2, ux, uy, uz;
             rpy_coef(dx, dy, dz, ca[gi], ca[gj], kp.p1, t1, t2, ux, uy, uz);
@@ -658,7 +697,7 @@ __global__ void __launch_bounds__(256) k_build_blocks(const BlkTask *__restrict_
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int b = 0; b < 3; b++)
-                    A[(long long)(3 * i + a) * ld + 3 * j + b] = t2 * u[a] * u[b] + (a == b ? t1 : 0.0);
+                    A[(long long)(3 * i + a) * ld + 3 * j + b] = (TS)(t2 * u[a] * u[b] + (a == b ? t1 : 0.0));
         }
     }
 }

@@ -1,0 +1,187 @@
+// Multi-RHS H2 product (H2P_matmul, src/H2Pack_matmul.c:455-583) on MFMA.
+//
+// Every phase of the matmul is a list of tasks of one form,
+//     dst (M x nv)  (+)=  sum_s  op(A_s) (M x K_s) * X_s (K_s x nv),
+// evaluated in the pull formulation of H2P_matmul_intmd_mult /
+// H2P_matmul_dense_mult (one task per destination, each source block applied
+// to the destination side only), so every output row has one writer and no
+// partial buffers are needed:
+//   upward   : Y0[node]      = U^T[node] * (PX[leaf rows] | Y0[children])
+//   coupling : Y1[node]      = sum op(B) * (Y0[other] | PX[other leaf rows])
+//   downward : Y1[children] += U[node] * Y1[node]           (level by level)
+//   leaves   : PY[leaf rows] = U[leaf] * Y1[leaf] + sum op(D) * PX + sum op(B) * Y0
+// A_s is a dense block (U, U^T, or a stored B/D block, transposed for the
+// negative entries of the pair-to-block CSR) or, in JIT mode, kernel values
+// evaluated into the tile: the kernels are symmetric, so op(B) = K(dst set,
+// src set).  Panels are row-major with a row length of nvp (vectors padded to
+// a multiple of 16).  The products run on v_mfma_f64_16x16x4f64: lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
+// D[4 r + l / 16][l % 16] in its register r.
+#pragma once
+#include "h2p_device.h"
+
+namespace h2p {
+
+enum { MM_PX = 0, MM_Y0 = 1, MM_Y1 = 2, MM_PY = 3 };
+enum { MM_A_U = 0, MM_A_UT = 1, MM_A_BD = 2, MM_A_JIT = 3 };
+
+struct MMSrc {
+    long long a_off;   // dense: element offset of A in its blob
+    int kind;          // MM_A_*
+    int trans;         // dense: op(A) = A^T
+    int lda;           // dense: row length of the stored A
+    int K;             // inner dimension (vector entries)
+    int c0, c1;        // JIT: first destination / source point (unified coordinates)
+    int x_space, x_off;
+};
+
+struct MMTask {
+    int dst_space, dst_off, M, accumulate;
+    int sbeg, send, pad0, pad1;
+};
+
+struct MMPanels {
+    double *p[4];      // PX, Y0, Y1, PY
+    int nvp;           // padded row length
+};
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Tile geometry: KD = 1: 64 rows x 16 inner; KD = 3: 48 rows (16 points) x 48 inner.
+template <int KD> struct MMGeom;
+template <> struct MMGeom<1> { static constexpr int RC = 64, KC = 16; };
+template <> struct MMGeom<3> { static constexpr int RC = 48, KC = 48; };
+
+#define H2P_MM_MAXNVP 64
+
+template <int KID, int KD>
+__global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
+                                            const MMSrc *__restrict__ srcs,
+                                            const double *__restrict__ Ublob,
+                                            const double *__restrict__ UTblob,
+                                            const double *__restrict__ BDblob,
+                                            const double *__restrict__ cx, long long cstride,
+                                            KParam kp, MMPanels P) {
+    constexpr int RC = MMGeom<KD>::RC, KC = MMGeom<KD>::KC;
+    constexpr int MT = RC / 16;                 // M tiles per chunk (4 or 3)
+    __shared__ double As[RC][KC + 1];
+    __shared__ double Xs[KC][H2P_MM_MAXNVP + 1];
+    const MMTask t = tasks[blockIdx.x];
+    const int nvp = P.nvp, NT = nvp >> 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lq = lane >> 4;
+    double *dst = P.p[t.dst_space] + (long long)t.dst_off * nvp;
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+    for (int mc = 0; mc < t.M; mc += RC) {
+        d4 acc[H2P_MM_MAXNVP / 16];
+#pragma unroll
+        for (int q = 0; q < H2P_MM_MAXNVP / 16; q++) acc[q] = (d4){0.0, 0.0, 0.0, 0.0};
+        for (int si = t.sbeg; si < t.send; si++) {
+            const MMSrc s = srcs[si];
+            const double *X = P.p[s.x_space] + (long long)s.x_off * nvp;
+            for (int kc = 0; kc < s.K; kc += KC) {
+                __syncthreads();
+                // ---- A tile (RC x KC), zero outside the block
+                if (s.kind == MM_A_JIT) {
+                    constexpr int RP = RC / KD, CP = KC / KD;
+                    const int p0 = mc / KD, q0 = kc / KD;
+                    const int np0 = t.M / KD - p0, nq0 = s.K / KD - q0;
+                    for (int e = threadIdx.x; e < RP * CP; e += blockDim.x) {
+                        const int i = e / CP, j = e - i * CP;
+                        const bool ok = i < np0 && j < nq0;
+                        const int gi = s.c0 + p0 + (ok ? i : 0), gj = s.c1 + q0 + (ok ? j : 0);
+                        const double dx = cx[gi] - cx[gj], dy = cy[gi] - cy[gj], dz = cz[gi] - cz[gj];
+                        if constexpr (KD == 1) {
+                            As[i][j] = ok ? kval<KID>(dx * dx + dy * dy + dz * dz, kp) : 0.0;
+                        } else {
+                            double t1, t2, ux, uy, uz;
+                            rpy_coef(dx, dy, dz, ca[gi], ca[gj], kp.p1, t1, t2, ux, uy, uz);
+                            const double u[3] = {ux, uy, uz};
+#pragma unroll
+                            for (int a = 0; a < 3; a++)
+#pragma unroll
+                                for (int b = 0; b < 3; b++)
+                                    As[3 * i + a][3 * j + b] = ok ? t2 * u[a] * u[b] + (a == b ? t1 : 0.0) : 0.0;
+                        }
+                    }
+                } else {
+                    const double *A = (s.kind == MM_A_U ? Ublob : s.kind == MM_A_UT ? UTblob : BDblob) + s.a_off;
+                    if (!s.trans) {
+                        for (int e = threadIdx.x; e < RC * KC; e += blockDim.x) {
+                            const int i = e / KC, j = e - i * KC;
+                            const int m = mc + i, k = kc + j;
+                            As[i][j] = (m < t.M && k < s.K) ? A[(long long)m * s.lda + k] : 0.0;
+                        }
+                    } else {
+                        for (int e = threadIdx.x; e < RC * KC; e += blockDim.x) {
+                            const int j = e / RC, i = e - j * RC;
+                            const int m = mc + i, k = kc + j;
+                            As[i][j] = (m < t.M && k < s.K) ? A[(long long)k * s.lda + m] : 0.0;
+                        }
+                    }
+                }
+                // ---- X tile (KC x nvp), zero past K
+                for (int e = threadIdx.x; e < KC * nvp; e += blockDim.x) {
+                    const int k = e / nvp, c = e - k * nvp;
+                    Xs[k][c] = (kc + k < s.K) ? X[(long long)(kc + k) * nvp + c] : 0.0;
+                }
+                __syncthreads();
+                if (w < MT) {
+#pragma unroll
+                    for (int kk = 0; kk < KC; kk += 4) {
+                        const double a = As[w * 16 + li][kk + lq];
+#pragma unroll
+                        for (int q = 0; q < H2P_MM_MAXNVP / 16; q++)
+                            if (q < NT)
+                                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Xs[kk + lq][q * 16 + li], acc[q], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // ---- write the chunk: register j of lane (li, lq) holds row 4 j + lq of
+        // the M tile, column li (the f64 16x16x4 output layout)
+        if (w < MT) {
+#pragma unroll
+            for (int q = 0; q < H2P_MM_MAXNVP / 16; q++) {
+                if (q >= NT) break;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int m = mc + w * 16 + 4 * j + lq;
+                    if (m < t.M) {
+                        double *o = dst + (long long)m * nvp + q * 16 + li;
+                        *o = t.accumulate ? *o + acc[q][j] : acc[q][j];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Caller panel (original order, row- or column-major, ld) -> PX (permuted,
+// row-major nvp, zero padding), for columns [v0, v0 + nv).
+__global__ void k_mm_gather(const double *__restrict__ X, long long ldx, int row_major, int v0, int nv,
+                            const int *__restrict__ fwd, int N, double *__restrict__ PX, int nvp) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)N * nvp) return;
+    const int i = (int)(e / nvp), c = (int)(e - (long long)i * nvp);
+    double v = 0.0;
+    if (c < nv) {
+        const long long src = fwd[i];
+        v = row_major ? X[src * ldx + v0 + c] : X[(long long)(v0 + c) * ldx + src];
+    }
+    PX[e] = v;
+}
+
+// PY -> caller panel: Y[fwd[i], v0 + c] = PY[i, c] (the backward permutation).
+__global__ void k_mm_scatter(const double *__restrict__ PY, int nvp, const int *__restrict__ fwd, int N,
+                             double *__restrict__ Y, long long ldy, int row_major, int v0, int nv) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)N * nv) return;
+    const int i = (int)(e / nv), c = (int)(e - (long long)i * nv);
+    const long long dstr = fwd[i];
+    const double v = PY[(long long)i * nvp + c];
+    if (row_major) Y[dstr * ldy + v0 + c] = v;
+    else Y[(long long)(v0 + c) * ldy + dstr] = v;
+}
+
+}  // namespace h2p

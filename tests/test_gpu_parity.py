@@ -45,18 +45,43 @@ def test_matvec_is_deterministic(name):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("name", ["gaussian", "rpy"])
+@pytest.mark.parametrize("name", ["coulomb_diag", "gaussian", "rpy"])
+@pytest.mark.parametrize("mode", ["jit", "aot"])
 @pytest.mark.parametrize("layout", ["col", "row"])
-def test_matmul_matches_oracle_and_matvec(name, layout):
+def test_matmul_matches_oracle_and_matvec(name, mode, layout):
+    """MFMA pull-form matmul: against the oracle's H2P_matmul restatement and
+    against one matvec per column (extra/test_H2_matmul.h:18-116)."""
     h2, _ = problem(name)
     rng = np.random.default_rng(5)
     X = rng.standard_normal((h2.krnl_mat_size, 7))
-    m = H2MatHIP(h2, mode="jit")
+    m = H2MatHIP(h2, mode=mode, host_blocks=False)
     Y = m.matmul(X, layout)
     Y_ref = O.OracleH2(h2, BD_JIT=1).matmul(X, layout, n_thread=8)
-    assert rel(Y, Y_ref) < FP64_TOL
+    err = rel(Y, Y_ref)
+    print(f"[{name} {mode} {layout}] matmul rel err vs oracle = {err:.3e}", flush=True)
+    assert err < FP64_TOL
     cols = np.stack([m.matvec(X[:, i]) for i in range(X.shape[1])], axis=1)
     assert rel(Y, cols) < FP64_TOL
+
+
+def test_matmul_many_columns_and_leading_dimensions():
+    """70 columns cross the 64-column chunk; ld larger than the panel."""
+    import ctypes as ct
+    h2, _ = problem("gaussian")
+    N = h2.krnl_mat_size
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((N, 70))
+    m = H2MatHIP(h2, mode="jit")
+    Y_ref = O.OracleH2(h2, BD_JIT=1).matmul(X, "col", n_thread=8)
+    assert rel(m.matmul(X, "col"), Y_ref) < FP64_TOL
+    # column-major with ldx = N + 5, through the C ABI directly
+    ld = N + 5
+    Xb = np.zeros((70, ld)); Xb[:, :N] = X.T
+    Yb = np.full((70, ld), 7.0)
+    rc = m.lib.h2p_hip_matmul(m.handle, abi.COL_MAJOR, 70, Xb.ctypes.data, ld, Yb.ctypes.data, ld)
+    assert rc == 0
+    assert rel(Yb[:, :N].T, Y_ref) < FP64_TOL
+    assert np.all(Yb[:, N:] == 7.0)          # padding untouched
 
 
 @pytest.mark.parametrize("name", ["coulomb", "gaussian", "rpy"])
@@ -94,18 +119,29 @@ def test_bad_calls_raise():
 
 
 @pytest.mark.parametrize("name", ["coulomb", "coulomb_diag", "gaussian", "matern32"])
-def test_matvec_fp32mix_within_build_tolerance(name):
-    """FP32 kernel values and block products, FP64 U sweeps and outputs:
-    error against the FP64 oracle stays within the H2 build tolerance."""
+@pytest.mark.parametrize("mode", ["jit", "aot", "aot_devbuild"])
+def test_matvec_fp32mix_within_build_tolerance(name, mode):
+    """FP32 kernel values (or FP32-stored blocks) and block products, FP64 U
+    sweeps and outputs: error against the FP64 oracle stays within the H2
+    build tolerance."""
     h2, _ = problem(name)
     x = _x(h2)
     y_ref = O.OracleH2(h2, BD_JIT=1).matvec(x, n_thread=8)
-    m = H2MatHIP(h2, mode="jit", prec="fp32mix")
+    m = H2MatHIP(h2, mode="jit" if mode == "jit" else "aot", prec="fp32mix",
+                 host_blocks=(mode == "aot"))
     y = m.matvec(x)
     err = rel(y, y_ref)
-    print(f"[{name} fp32mix] rel err vs FP64 oracle = {err:.3e}", flush=True)
+    print(f"[{name} {mode} fp32mix] rel err vs FP64 oracle = {err:.3e}", flush=True)
     assert err < h2.QR_stop_tol
     assert np.array_equal(y, m.matvec(x))   # deterministic
+
+
+def test_rpy_aot_fp32mix_within_build_tolerance():
+    h2, _ = problem("rpy")
+    x = _x(h2)
+    y_ref = O.OracleH2(h2, BD_JIT=1).matvec(x, n_thread=8)
+    y = H2MatHIP(h2, mode="aot", prec="fp32mix", host_blocks=False).matvec(x)
+    assert rel(y, y_ref) < h2.QR_stop_tol
 
 
 def test_fp32mix_refused_where_unavailable():
