@@ -66,6 +66,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--check-rows", type=int, default=1000)
+    ap.add_argument("--workload", default="gaussian1m",
+                    choices=["gaussian1m", "coulomb80k", "rpy_matmul", "precision2m"],
+                    help="gaussian1m: the headline (default); the others are the remaining "
+                         "configurations, each printed as its own JSON line")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 2 GiB device-to-device copy first, the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -78,6 +82,11 @@ def main():
     import torch
     from h2pack_amd import construct as C
     from h2pack_amd.h2mat import H2MatHIP, direct_nbody
+
+    if args.workload != "gaussian1m":
+        import bench_workloads
+        bench_workloads.run(args)
+        return
 
     N, K, W = args.n, args.steps, args.warmup
     sigma2 = 1e-2

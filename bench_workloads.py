@@ -1,0 +1,161 @@
+"""The non-headline configurations of bench.py (`--workload ...`).
+
+coulomb80k   3D Coulomb, N = 80,000 uniform points in a cube of side N^(1/3),
+             reltol 1e-6, leaf 300, AOT (stored B and D), one FP64 RHS
+             U(-0.5, 0.5); GPU matvec vs the oracle's AOT matvec on this host.
+rpy_matmul   RPY tensor kernel (a = 1, eta = 1), N = 300,000 particles
+             uniform at 20% volume fraction, reltol 1e-7, leaf 200, H2P_matmul
+             with 32 FP64 columns ~ N(0, 1), column-major; JIT and AOT.
+precision2m  3D Gaussian (l = 1) and 3D Coulomb at N = 2,000,000, density 10,
+             reltol 1e-6, leaf 300, JIT: FP64 vs FP32-mixed matvec.
+Every figure is on one GPU with vectors resident in HBM.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _time(fn, K, W, sync):
+    for _ in range(W):
+        fn()
+    sync()
+    t = time.perf_counter()
+    for _ in range(K):
+        fn()
+    sync()
+    return (time.perf_counter() - t) / K
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run(args):
+    import torch
+    from h2pack_amd import abi
+    from h2pack_amd import construct as C
+    from h2pack_amd.h2mat import H2MatHIP, direct_nbody
+    dev = torch.device("cuda:0")
+    sptr = torch.cuda.current_stream().cuda_stream
+    sync = torch.cuda.synchronize
+    K, W = max(1, args.steps), max(1, args.warmup)
+    out = {"workload": args.workload, "n_gpus": 1, "steps": K, "warmup": W, "data": "synthetic"}
+
+    if args.workload == "coulomb80k":
+        N = 80_000
+        coord = C.uniform_cube_points(N, 1.0, args.seed)
+        t0 = time.perf_counter()
+        h2 = C.build_h2(coord, C.KRNL_COULOMB, [], 1e-6, 300, BD_JIT=0, pp_kind="numerical", host_blocks=True)
+        tb = time.perf_counter() - t0
+        x = np.random.default_rng(args.seed + 1).random(N) - 0.5
+        xd = torch.from_numpy(x).to(dev); yd = torch.empty_like(xd)
+        m = H2MatHIP(h2, mode="aot", host_blocks=True)
+        dt = _time(lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr), K, W, sync)
+        info = m.info()
+        y = yd.cpu().numpy()
+        rows = np.sort(np.random.default_rng(args.seed + 2).choice(N, 1000, replace=False))
+        yref = direct_nbody(C.KRNL_COULOMB, [], coord, x, rows)
+        out.update({"metric": "H2 matvecs/s (Coulomb N=80k AOT, FP64)", "value": 1.0 / dt, "unit": "matvecs/s",
+                    "ms_per_step": dt * 1e3, "dtype": "f64",
+                    "hbm_GBps_model": info["hbm_bytes_model"] / dt / 1e9,
+                    "rel_err_vs_direct_nbody_1000rows": _rel(y[rows], yref), "build_s": tb})
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            o = O.OracleH2(h2, BD_JIT=0)
+            nt = os.cpu_count() or 1
+            o.matvec(x, n_thread=nt)
+            t = time.perf_counter(); calls = 0
+            while calls < 10 and time.perf_counter() - t < 10:
+                yo = o.matvec(x, n_thread=nt); calls += 1
+            el = (time.perf_counter() - t) / calls
+            out["cpu_baseline"] = {"value": 1.0 / el, "unit": "matvecs/s", "cores": nt, "kind": "port",
+                                   "sample": f"{calls} AOT FP64 matvecs of the same H2 matrix (oracle, OpenMP)"}
+            out["rel_err_vs_oracle"] = _rel(y, yo)
+
+    elif args.workload == "rpy_matmul":
+        N, nv = 300_000, 32
+        L = (N * (4.0 / 3.0) * math.pi / 0.2) ** (1.0 / 3.0)
+        rng = np.random.default_rng(args.seed)
+        coord = np.concatenate([rng.random((3, N)) * L, np.ones((1, N))], axis=0)
+        t0 = time.perf_counter()
+        h2 = C.build_h2(coord, C.KRNL_RPY, [1.0], 1e-7, 200, BD_JIT=1, pp_kind="surface", max_leaf_size=4.0)
+        tb = time.perf_counter() - t0
+        log(f"[bench] RPY N={N}: built in {tb:.1f}s, levels {h2.max_level + 1}, adm {h2.n_r_adm_pair}")
+        Nk = h2.krnl_mat_size
+        X = np.asfortranarray(rng.standard_normal((Nk, nv)))
+        Xd = torch.from_numpy(X.T.copy()).to(dev)          # column-major: column v at v * Nk
+        Yd = torch.empty_like(Xd)
+        res = {}
+        for mode in ("jit", "aot"):
+            m = H2MatHIP(h2, mode=mode, host_blocks=False)
+            dt = _time(lambda: m.matmul_dev(abi.COL_MAJOR, nv, Xd.data_ptr(), Nk, Yd.data_ptr(), Nk, sptr), K, W, sync)
+            Y = Yd.cpu().numpy().T
+            xv = torch.empty(Nk, dtype=torch.float64, device=dev); yv = torch.empty_like(xv)
+            def cols():
+                for v in range(nv):
+                    m.matvec_dev(Xd[v].data_ptr(), yv.data_ptr(), sptr)
+            dtv = _time(cols, 1, 1, sync)
+            # matmul == matvec per column
+            errs = []
+            for v in (0, nv - 1):
+                m.matvec_dev(Xd[v].data_ptr(), yv.data_ptr(), sptr); sync()
+                errs.append(_rel(Y[:, v], yv.cpu().numpy()))
+            info = m.info()
+            flops = 2.0 * nv * (2 * info["U_elems"] + 2 * 9 * info["B_pairs"] + 2 * 9 * info["D_pairs"])
+            res[mode] = {"matmuls_per_s": 1.0 / dt, "ms_per_matmul": dt * 1e3,
+                         "ms_per_32_matvecs": dtv * 1e3, "speedup_vs_matvec_loop": dtv / dt,
+                         "model_TFLOPs": flops / dt / 1e12, "rel_err_vs_matvec_cols": max(errs)}
+            log(f"[bench] rpy {mode}: {dt * 1e3:.2f} ms per 32-column matmul ({flops / dt / 1e12:.2f} TFLOP/s model), "
+                f"32 matvecs {dtv * 1e3:.2f} ms")
+            m.close()
+        rows = np.sort(rng.choice(N, 1000, replace=False))
+        yref = direct_nbody(C.KRNL_RPY, [1.0], coord, np.ascontiguousarray(X[:, 0]), rows)
+        Yr = Y[:, 0].reshape(N, 3)[rows].reshape(-1)
+        out.update({"metric": "H2 matmuls/s (RPY N=300k, 32 RHS, FP64)", "value": res["jit"]["matmuls_per_s"],
+                    "unit": "matmuls/s", "dtype": "f64", "jit": res["jit"], "aot": res["aot"],
+                    "rel_err_vs_direct_nbody_1000rows_col0": _rel(Yr, yref), "build_s": tb})
+
+    else:   # precision2m
+        N = 2_000_000
+        res = {}
+        for kid, name, param in ((C.KRNL_GAUSSIAN, "gaussian", [1.0]), (C.KRNL_COULOMB, "coulomb", [])):
+            coord = C.uniform_cube_points(N, 10.0, args.seed)
+            t0 = time.perf_counter()
+            h2 = C.build_h2(coord, kid, param, 1e-6, 300, BD_JIT=1, pp_kind="numerical")
+            tb = time.perf_counter() - t0
+            x = np.random.default_rng(args.seed + 1).random(N) - 0.5
+            xd = torch.from_numpy(x).to(dev); yd = torch.empty_like(xd)
+            ys, ts = {}, {}
+            for prec in ("fp64", "fp32mix"):
+                m = H2MatHIP(h2, mode="jit", prec=prec)
+                ts[prec] = _time(lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr), K, W, sync)
+                ys[prec] = yd.cpu().numpy()
+                m.close()
+            rows = np.sort(np.random.default_rng(args.seed + 2).choice(N, 1000, replace=False))
+            yref = direct_nbody(kid, param, coord, x, rows)
+            res[name] = {"fp64_matvecs_per_s": 1 / ts["fp64"], "fp32mix_matvecs_per_s": 1 / ts["fp32mix"],
+                         "fp32mix_speedup": ts["fp64"] / ts["fp32mix"],
+                         "fp32mix_rel_err_vs_fp64_gpu": _rel(ys["fp32mix"], ys["fp64"]),
+                         "fp64_rel_err_vs_direct_nbody_1000rows": _rel(ys["fp64"][rows], yref),
+                         "fp32mix_rel_err_vs_direct_nbody_1000rows": _rel(ys["fp32mix"][rows], yref),
+                         "build_s": tb, "levels": h2.max_level + 1, "adm_pairs": h2.n_r_adm_pair}
+            log(f"[bench] {name} 2M: fp64 {1 / ts['fp64']:.2f}/s fp32mix {1 / ts['fp32mix']:.2f}/s "
+                f"errs {res[name]['fp32mix_rel_err_vs_fp64_gpu']:.2e} {res[name]['fp64_rel_err_vs_direct_nbody_1000rows']:.2e}")
+            del h2
+        out.update({"metric": "H2 matvecs/s at N=2M, FP64 vs FP32-mixed (Gaussian, Coulomb)",
+                    "value": res["gaussian"]["fp32mix_matvecs_per_s"], "unit": "matvecs/s",
+                    "dtype": "f32-mixed", "gaussian": res["gaussian"], "coulomb": res["coulomb"]})
+    print(json.dumps(out), flush=True)
