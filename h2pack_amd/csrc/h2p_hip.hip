@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -110,6 +111,11 @@ struct h2p_hip_handle {
     int ev_used = 0;      // calls recorded since the last read
     int ev_cap = 0;       // calls the pool holds
     hipStream_t own_stream = nullptr;
+    // the interior of a matvec (everything between the gather of x and the
+    // leaf outputs) touches only handle-owned buffers: captured once, replayed
+    int use_graph = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 static void free_handle(h2p_hip_handle *h) {
@@ -120,6 +126,8 @@ static void free_handle(h2p_hip_handle *h) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
 }
@@ -165,15 +173,12 @@ static void dispatch_build(h2p_hip_handle *h, hipStream_t s) {
     }
 }
 
-// One matvec on device pointers (x, y in the caller's order).
-static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
-    const bool prof = h->profiling && h->ev_used < h->ev_cap;
-    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * 6] : nullptr;
-    if (prof) HIP_TRY(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
+// Interior of a matvec: upward sweep, grouped B/D launch, skeleton sums,
+// downward sweep (reads xin[0, N) = px, writes yp / y1 / arena).
+static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
     for (const Level &L : h->fwd_levels)
         hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
-    if (prof) HIP_TRY(hipEventRecord(ev[1], s));
+    if (ev) (void)hipEventRecord(ev[1], s);
     if (h->n_blk > 0) {
         if (h->mode == H2P_HIP_AOT && h->prec == H2P_HIP_FP32MIX)
             hipLaunchKernelGGL(k_blocks_aot<float>, dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
@@ -184,13 +189,41 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
         else
             dispatch_jit(h, s);
     }
-    if (prof) HIP_TRY(hipEventRecord(ev[2], s));
+    if (ev) (void)hipEventRecord(ev[2], s);
     if (h->n_red > 0)
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
-    if (prof) HIP_TRY(hipEventRecord(ev[3], s));
+    if (ev) (void)hipEventRecord(ev[3], s);
     for (const Level &L : h->bwd_levels)
         hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
-    if (prof) HIP_TRY(hipEventRecord(ev[4], s));
+    if (ev) (void)hipEventRecord(ev[4], s);
+}
+
+static int ensure_graph(h2p_hip_handle *h) {
+    if (h->graph_exec) return 0;
+    hipStream_t cs = h->own_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    launch_interior(h, cs, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    if (e != hipSuccess) return fail(H2P_HIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    h->graph = g;
+    HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    return 0;
+}
+
+// One matvec on device pointers (x, y in the caller's order).
+static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
+    const bool prof = h->profiling && h->ev_used < h->ev_cap;
+    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * 6] : nullptr;
+    if (prof) HIP_TRY(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
+    if (!prof && h->use_graph) {
+        int rc = ensure_graph(h);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(h->graph_exec, s));
+    } else {
+        launch_interior(h, s, ev);
+    }
     hipLaunchKernelGGL(k_leaf_final2, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
                        h->d_UT, h->d_yp, h->d_fwd, d_y);
     HIP_TRY(hipGetLastError());
@@ -761,6 +794,10 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         }
     }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "stream create failed"));
+    {
+        const char *ng = getenv("H2P_HIP_NO_GRAPH");
+        h->use_graph = (ng && ng[0] == '1') ? 0 : 1;
+    }
     // ---- statistics (H2Pack's mat_size / JIT_flops model)
     h2p_hip_info &I = h->info;
     I.U_elems = (size_t)utot;

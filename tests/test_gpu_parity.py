@@ -149,3 +149,26 @@ def test_fp32mix_refused_where_unavailable():
     with pytest.raises(H2PHipError) as ei:
         H2MatHIP(h2, mode="jit", prec="fp32mix")
     assert ei.value.code == abi.ENOSYS
+
+
+@pytest.mark.parametrize("mode", ["jit", "aot"])
+def test_graph_replays_match_eager_path(mode):
+    """The matvec interior is a captured hipGraph replayed on every call; the
+    profiled path launches the same kernels eagerly.  Several replays with
+    different inputs and unrelated device work in between must each equal
+    the eager result bit for bit."""
+    import torch
+    h2, _ = problem("gaussian")
+    m = H2MatHIP(h2, mode=mode, host_blocks=False)
+    rng = np.random.default_rng(21)
+    junk = torch.randn(1 << 20, device="cuda:0", dtype=torch.float64)
+    for it in range(4):
+        x = rng.random(h2.krnl_mat_size) - 0.5
+        y_graph = m.matvec(x)
+        junk.mul_(1.0001).add_(0.5)          # eager work between replays
+        torch.cuda.synchronize()
+        m.set_profiling(True)                # profiled calls do not use the graph
+        y_eager = m.matvec(x)
+        m.info()
+        m.set_profiling(False)
+        assert np.array_equal(y_graph, y_eager), it
