@@ -8,6 +8,11 @@ rpy_matmul   RPY tensor kernel (a = 1, eta = 1), N = 300,000 particles
              with 32 FP64 columns ~ N(0, 1), column-major; JIT and AOT.
 precision2m  3D Gaussian (l = 1) and 3D Coulomb at N = 2,000,000, density 10,
              reltol 1e-6, leaf 300, JIT: FP64 vs FP32-mixed matvec.
+serving      Poisson arrivals of Matern-3/2 GP requests (lengthscale U(0.5, 2),
+             nugget 1e-2, N log-uniform in [1e4, 4e5], density 10, reltol
+             1e-6, leaf 300): half 8-RHS products, half CG solves to 1e-6;
+             continuous batching on the FP32-mixed path against serving each
+             request alone on the FP64 path (--requests of them).
 Every figure is on one GPU with vectors resident in HBM.
 """
 from __future__ import annotations
@@ -42,7 +47,90 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def _build_worker(spec, kw, threads):
+    os.environ["H2P_BUILD_THREADS"] = str(threads)
+    from threadpoolctl import threadpool_limits
+    from h2pack_amd.serve import build_request
+    with threadpool_limits(limits=threads):
+        t = time.perf_counter()
+        r = build_request(spec, **kw)
+        return r, time.perf_counter() - t
+
+
+def run_serving(args):
+    """Matern-3/2 GP requests through the continuous-batching loop (FP32-mixed
+    JIT, co-scheduled launches) and, on the same arrivals, one request at a
+    time on the plain FP64 path.  Host construction happens before the clock
+    starts, in worker processes that never touch the GPU."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from h2pack_amd.serve import Server, request_specs
+    specs = request_specs(args.requests, args.seed, 10_000, 400_000, args.rate)
+    kw = {"max_iter": args.cg_max_iter}
+    workers = max(1, args.build_workers)
+    threads = max(1, 16 // workers)
+    t0 = time.perf_counter()
+    reqs = [None] * len(specs)
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        futs = {ex.submit(_build_worker, s, kw, threads): s["rid"] for s in specs}
+        for f in futs:
+            r, tb = f.result()
+            reqs[futs[f]] = r
+            log(f"[bench] request {r.rid}: {r.kind} N={r.N} built in {tb:.1f}s")
+    t_build = time.perf_counter() - t0
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    budget = int(0.8 * free)
+
+    def fresh():
+        import copy
+        out = []
+        for r in reqs:
+            c = copy.copy(r)
+            c.result = None
+            out.append(c)
+        return out
+
+    res = {}
+    for name, prec, slots in (("batched_fp32mix", "fp32mix", args.slots), ("alone_fp64", "fp64", 1)):
+        rs = fresh()
+        srv = Server(hbm_budget=budget, max_active=slots, prec=prec, cg_chunk=8)
+        st = srv.run(rs, realtime=True)
+        srv.close()
+        cg = [r for r in rs if r.kind == "cg"]
+        d = st.as_dict()
+        d["cg_converged"] = sum(1 for r in cg if r.relres <= r.tol)
+        d["cg_requests"] = len(cg)
+        d["cg_mean_iters"] = float(np.mean([r.iters for r in cg])) if cg else 0.0
+        res[name] = (d, rs)
+        log(f"[bench] {name}: {st.n_done} done in {st.makespan:.1f}s, {st.throughput:.3f} req/s, "
+            f"p50 {st.p50:.2f}s p99 {st.p99:.2f}s, max active {st.max_active}")
+    # the FP32-mixed results against the FP64 ones
+    errs = []
+    for a, b in zip(res["batched_fp32mix"][1], res["alone_fp64"][1]):
+        if a.kind == "matmul":
+            errs.append(_rel(a.result, b.result))
+    d32, d64 = res["batched_fp32mix"][0], res["alone_fp64"][0]
+    out = {"metric": "served GP requests/s and p50/p99 latency (Matern-3/2, FP32-mixed batched vs FP64 alone)",
+           "value": d32["throughput"], "unit": "requests/s", "higher_is_better": True,
+           "n_gpus": 1, "dtype": "f32-mixed", "data": "synthetic",
+           "config": {"workload": f"serving_matern32_{args.requests}req_N1e4-4e5_density10_reltol1e-6_leaf300"
+                                  f"_rate{args.rate}_8rhs_cg_tol1e-6_nugget1e-2",
+                      "requests": args.requests, "rate_per_s": args.rate, "slots": args.slots,
+                      "hbm_budget_bytes": budget, "cg_max_iter": args.cg_max_iter},
+           "batched_fp32mix": d32, "alone_fp64": d64,
+           "throughput_speedup": d32["throughput"] / d64["throughput"] if d64["throughput"] else None,
+           "p50_speedup": d64["p50"] / d32["p50"] if d32["p50"] else None,
+           "p99_speedup": d64["p99"] / d32["p99"] if d32["p99"] else None,
+           "matmul_max_rel_diff_fp32mix_vs_fp64": max(errs) if errs else None,
+           "host_build_s": t_build, "mean_N": float(np.mean([r.N for r in reqs])),
+           "vs_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def run(args):
+    if args.workload == "serving":
+        return run_serving(args)
     import torch
     from h2pack_amd import abi
     from h2pack_amd import construct as C

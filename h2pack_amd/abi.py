@@ -12,7 +12,10 @@ EINVAL, ENOMEM, EDEVICE, ENOSYS = -1, -2, -3, -4
 # every symbol include/h2p_hip.h declares
 EXPORTS = ("h2p_hip_upload", "h2p_hip_matvec", "h2p_hip_matvec_dev", "h2p_hip_matmul",
            "h2p_hip_matmul_dev", "h2p_hip_set_profiling", "h2p_hip_get_info",
-           "h2p_hip_free", "h2p_hip_last_error", "h2p_hip_direct_nbody")
+           "h2p_hip_free", "h2p_hip_last_error", "h2p_hip_direct_nbody",
+           "h2p_hip_footprint", "h2p_hip_batch_create", "h2p_hip_batch_free",
+           "h2p_hip_matvec_batch", "h2p_hip_matmul_batch", "h2p_hip_cg_start",
+           "h2p_hip_cg_step", "h2p_hip_cg_poll")
 
 _vp = ct.c_void_p
 
@@ -80,4 +83,22 @@ def bind(lib: ct.CDLL) -> ct.CDLL:
     lib.h2p_hip_direct_nbody.argtypes = [ct.POINTER(KernelDesc), ct.c_int, _vp, ct.c_int, ct.c_int,
                                          _vp, _vp, ct.c_int, ct.c_int, _vp]
     lib.h2p_hip_direct_nbody.restype = ct.c_int
+    lib.h2p_hip_footprint.argtypes = [ct.POINTER(H2PackDesc), ct.POINTER(KernelDesc), ct.c_int, ct.c_int,
+                                      ct.c_int, ct.c_int, ct.POINTER(ct.c_size_t)]
+    lib.h2p_hip_footprint.restype = ct.c_int
+    lib.h2p_hip_batch_create.argtypes = [ct.POINTER(_vp)]
+    lib.h2p_hip_batch_create.restype = ct.c_int
+    lib.h2p_hip_batch_free.argtypes = [_vp]
+    lib.h2p_hip_batch_free.restype = None
+    # pointer arrays are passed as addresses of ctypes arrays
+    lib.h2p_hip_matvec_batch.argtypes = [_vp, ct.c_int, _vp, _vp, _vp, _vp, _vp]
+    lib.h2p_hip_matvec_batch.restype = ct.c_int
+    lib.h2p_hip_matmul_batch.argtypes = [_vp, ct.c_int, _vp, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.h2p_hip_matmul_batch.restype = ct.c_int
+    lib.h2p_hip_cg_start.argtypes = [_vp, ct.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.h2p_hip_cg_start.restype = ct.c_int
+    lib.h2p_hip_cg_step.argtypes = [_vp, ct.c_int, _vp, ct.c_int, _vp]
+    lib.h2p_hip_cg_step.restype = ct.c_int
+    lib.h2p_hip_cg_poll.argtypes = [_vp, ct.c_int, _vp, _vp, _vp, _vp, _vp]
+    lib.h2p_hip_cg_poll.restype = ct.c_int
     return lib

@@ -49,6 +49,9 @@ KRNL_DIM = {KRNL_COULOMB: 1, KRNL_GAUSSIAN: 1, KRNL_MATERN32: 1, KRNL_RPY: 3}
 
 
 def _nthreads() -> int:
+    env = os.environ.get("H2P_BUILD_THREADS")
+    if env:
+        return max(1, int(env))
     return max(1, min(16, os.cpu_count() or 1))
 
 

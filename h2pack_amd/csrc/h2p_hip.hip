@@ -24,6 +24,7 @@
 #include "h2p_hip.h"
 #include "h2p_kernels.h"
 #include "h2p_mm.h"
+#include "h2p_batch.h"
 
 using namespace h2p;
 
@@ -97,7 +98,8 @@ struct h2p_hip_handle {
     MMSrc *d_mms = nullptr;
     std::vector<Level> mm_fwd_levels, mm_bwd_levels;
     Level mm_intmd{0, 0}, mm_leaf{0, 0};
-    double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, H2P_MM_MAXNVP wide
+    double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, panel_nvp wide
+    int panel_nvp = 0;
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
@@ -116,13 +118,22 @@ struct h2p_hip_handle {
     int use_graph = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // device-side CG state of the batched solver (allocated by the first start)
+    double *d_cg = nullptr;       // r | p | s | partials | scalars
+    double *cg_r = nullptr, *cg_p = nullptr, *cg_s = nullptr, *cg_part = nullptr;
+    CGScal *cg_sc = nullptr;
+    double *cg_x = nullptr;
+    const double *cg_b = nullptr;
+    double cg_shift = 0.0, cg_tol = 0.0;
+    int cg_max_iter = 0, cg_nblk = 0, cg_started = 0;
 };
 
 static void free_handle(h2p_hip_handle *h) {
     if (!h) return;
     void *ptrs[] = {h->d_coord, h->d_U, h->d_UT, h->d_blob, h->d_xin, h->d_y1, h->d_arena,
                     h->d_fwd, h->d_blk, h->d_fwdt, h->d_bwdt, h->d_red, h->d_leaf,
-                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp, h->d_mmt, h->d_mms, h->d_panels};
+                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp, h->d_mmt, h->d_mms, h->d_panels,
+                    h->d_cg};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
@@ -225,30 +236,13 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
         launch_interior(h, s, ev);
     }
     hipLaunchKernelGGL(k_leaf_final2, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
-                       h->d_UT, h->d_yp, h->d_fwd, d_y);
+                       h->d_UT, h->d_yp, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
     HIP_TRY(hipGetLastError());
     if (prof) {
         HIP_TRY(hipEventRecord(ev[5], s));
         h->ev_used++;
     }
     return 0;
-}
-
-// column-major <-> row-major panel transposes for matmul staging
-__global__ void k_transpose(const double *__restrict__ in, long long ldi, double *__restrict__ out,
-                            long long ldo, int rows, int cols) {
-    // out[c * ldo + r] = in[r * ldi + c]
-    __shared__ double tile[32][33];
-    int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
-    for (int k = threadIdx.y; k < 32; k += 8) {
-        int r = r0 + k, c = c0 + threadIdx.x;
-        if (r < rows && c < cols) tile[k][threadIdx.x] = in[(long long)r * ldi + c];
-    }
-    __syncthreads();
-    for (int k = threadIdx.y; k < 32; k += 8) {
-        int c = c0 + k, r = r0 + threadIdx.x;
-        if (r < rows && c < cols) out[(long long)c * ldo + r] = tile[threadIdx.x][k];
-    }
 }
 
 template <int KID, int KD>
@@ -271,8 +265,12 @@ extern "C" {
 
 const char *h2p_hip_last_error(void) { return g_err.c_str(); }
 
-int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mode, int prec,
-                   h2p_hip_handle **out) {
+}  // extern "C"
+
+// Upload, or with dry != NULL only the validation and schedule building,
+// returning in *dry the device bytes the upload would hold.
+static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mode, int prec,
+                       h2p_hip_handle **out, size_t *dry) {
     if (!out) return fail(H2P_HIP_EINVAL, "out is NULL");
     *out = nullptr;
     if (!g || !krnl) return fail(H2P_HIP_EINVAL, "NULL representation or kernel");
@@ -727,6 +725,20 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
             if (!ok) return bail(fail(H2P_HIP_EINVAL, "matmul task out of range"));
         }
     }
+    if (dry) {
+        auto vb = [](const auto &v) {
+            return sizeof(typename std::decay_t<decltype(v)>::value_type) * std::max<size_t>(v.size(), 1);
+        };
+        size_t b = vb(cu) + vb(Ub) + vb(UTb) + sizeof(int) * std::max<size_t>(N, 1) + vb(blk) + vb(fwdt) +
+                   vb(bwdt) + vb(red) + vb(leaf) + vb(contrib) + vb(mmt) + vb(mms);
+        b += sizeof(double) * (std::max<size_t>((size_t)N + S, 1) + 2 * std::max<size_t>(S, 1) +
+                               std::max<size_t>((size_t)arena, 1));
+        if (mode == H2P_HIP_AOT)
+            b += (prec == H2P_HIP_FP32MIX ? sizeof(float) : sizeof(double)) * std::max<size_t>(btot + dtot, 1);
+        *dry = b;
+        free_handle(h);
+        return 0;
+    }
     // ---- device memory
     size_t *acct = &h->device_bytes;
     std::vector<int> fwdidx(g->fwd_pmt_idx, g->fwd_pmt_idx + N);
@@ -794,6 +806,9 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
         }
     }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "stream create failed"));
+    // the upload is synchronous: callers may use the handle on any stream,
+    // including streams that do not order with the null stream
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "upload did not complete"));
     {
         const char *ng = getenv("H2P_HIP_NO_GRAPH");
         h->use_graph = (ng && ng[0] == '1') ? 0 : 1;
@@ -820,6 +835,70 @@ int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mod
     I.mode = mode;
     I.prec = prec;
     *out = h;
+    return 0;
+}
+
+// matmul panel rows of a handle: PX (N) | Y0 (S) | Y1 (S) | PY (N)
+static size_t panel_rows(int N, int S) { return 2 * (size_t)N + 2 * (size_t)S; }
+static int panel_width(int n_vec) { return std::min(H2P_MM_MAXNVP, (n_vec + 15) & ~15); }
+
+static int ensure_panels(h2p_hip_handle *h, int nvp) {
+    if (h->panel_nvp >= nvp) return 0;
+    const size_t rows = panel_rows(h->N, h->S);
+    if (h->d_panels) {
+        // earlier products on any stream may still read the old panels
+        HIP_TRY(hipDeviceSynchronize());
+        (void)hipFree(h->d_panels);
+        h->d_panels = nullptr;
+        h->device_bytes -= sizeof(double) * rows * h->panel_nvp;
+        h->panel_nvp = 0;
+    }
+    if (hipMalloc(&h->d_panels, sizeof(double) * rows * nvp) != hipSuccess)
+        return fail(H2P_HIP_ENOMEM, "hipMalloc of the matmul panels failed");
+    h->panel_nvp = nvp;
+    h->device_bytes += sizeof(double) * rows * nvp;
+    h->info.device_bytes = h->device_bytes;
+    return 0;
+}
+
+static MMPanels panels_of(const h2p_hip_handle *h, int nvp) {
+    MMPanels P;
+    P.nvp = nvp;
+    P.p[MM_PX] = h->d_panels;
+    P.p[MM_Y0] = P.p[MM_PX] + (size_t)h->N * nvp;
+    P.p[MM_Y1] = P.p[MM_Y0] + (size_t)h->S * nvp;
+    P.p[MM_PY] = P.p[MM_Y1] + (size_t)h->S * nvp;
+    return P;
+}
+
+static int cg_blocks(int N) { return (N + H2P_CG_PER_BLOCK - 1) / H2P_CG_PER_BLOCK; }
+static size_t cg_doubles(int N) {
+    return 3 * (size_t)N + 2 * (size_t)cg_blocks(N) + (sizeof(CGScal) + 7) / 8;
+}
+
+extern "C" {
+
+int h2p_hip_upload(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mode, int prec,
+                   h2p_hip_handle **out) {
+    return upload_impl(g, krnl, mode, prec, out, nullptr);
+}
+
+int h2p_hip_footprint(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int mode, int prec, int n_vec,
+                      int with_cg, size_t *bytes) {
+    if (!bytes) return fail(H2P_HIP_EINVAL, "bytes is NULL");
+    *bytes = 0;
+    if (n_vec < 0) return fail(H2P_HIP_EINVAL, "n_vec < 0");
+    h2p_hip_handle *dummy = nullptr;
+    size_t b = 0;
+    int rc = upload_impl(g, krnl, mode, prec, &dummy, &b);
+    if (rc) return rc;
+    // S is needed for the panels: the sum of the U column counts
+    size_t S = 0;
+    for (int i = 0; i < g->n_node; i++) S += (size_t)g->U_ncol[i];
+    const int N = g->krnl_mat_size;
+    if (n_vec > 0) b += sizeof(double) * panel_rows(N, (int)S) * panel_width(n_vec);
+    if (with_cg) b += sizeof(double) * cg_doubles(N);
+    *bytes = b;
     return 0;
 }
 
@@ -858,24 +937,14 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
     if (rm ? (ldx < n_vec || ldy < n_vec) : (ldx < N || ldy < N)) return fail(H2P_HIP_EINVAL, "bad leading dimension");
     if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
     hipStream_t s = (hipStream_t)stream;
-    const size_t rows_total = 2 * (size_t)N + 2 * (size_t)h->S;
-    if (!h->d_panels) {
-        if (hipMalloc(&h->d_panels, sizeof(double) * rows_total * H2P_MM_MAXNVP) != hipSuccess)
-            return fail(H2P_HIP_ENOMEM, "hipMalloc of the matmul panels failed");
-        h->device_bytes += sizeof(double) * rows_total * H2P_MM_MAXNVP;
-        h->info.device_bytes = h->device_bytes;
-    }
+    int rc = ensure_panels(h, panel_width(n_vec));
+    if (rc) return rc;
     // columns in chunks of at most 64 (the reference chunks by mm_max_n_vec,
     // src/H2Pack_matmul.c:497-580)
     for (int v0 = 0; v0 < n_vec; v0 += H2P_MM_MAXNVP) {
         const int nv = std::min(H2P_MM_MAXNVP, n_vec - v0);
         const int nvp = (nv + 15) & ~15;
-        MMPanels P;
-        P.nvp = nvp;
-        P.p[MM_PX] = h->d_panels;
-        P.p[MM_Y0] = P.p[MM_PX] + (size_t)N * nvp;
-        P.p[MM_Y1] = P.p[MM_Y0] + (size_t)h->S * nvp;
-        P.p[MM_PY] = P.p[MM_Y1] + (size_t)h->S * nvp;
+        const MMPanels P = panels_of(h, nvp);
         const long long ne = (long long)N * nvp;
         hipLaunchKernelGGL(k_mm_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, d_X, (long long)ldx,
                            rm ? 1 : 0, v0, nv, h->d_fwd, N, P.p[MM_PX], nvp);
@@ -1008,6 +1077,564 @@ int h2p_hip_direct_nbody(const h2p_kernel_desc *krnl, int xpt_dim, const double 
     for (void *p : ps)
         if (p) (void)hipFree(p);
     return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Co-scheduled execution of many handles.
+//
+// Every call builds its launch tables on the host (a request table and, per
+// launch, the items that map workgroups to (request, task slice)), copies
+// them in one transfer from pinned staging to the device, and enqueues one
+// launch per phase for all requests.  Two staging slots alternate; a slot is
+// reused only after the launches that read it have finished.
+
+namespace {
+
+struct TableBuf {
+    std::vector<char> bytes;
+    template <typename T>
+    size_t add(const T *p, size_t n) {
+        size_t off = (bytes.size() + 15) & ~(size_t)15;
+        bytes.resize(off + sizeof(T) * std::max<size_t>(n, 1));
+        if (n) std::memcpy(bytes.data() + off, p, sizeof(T) * n);
+        return off;
+    }
+};
+
+// one launch over items of several requests
+struct BLaunch {
+    size_t items_off = 0;
+    int n_items = 0, grid = 0;
+    int mode = 0, prec = 0, kid = 0;   // kernel choice of the block launches
+};
+
+struct ItemList {
+    std::vector<BatchItem> items;
+    int total = 0;
+    void add(int req, const void *tasks, int count) {
+        if (count <= 0) return;
+        BatchItem it;
+        it.begin = total;
+        it.req = req;
+        it.tasks = tasks;
+        items.push_back(it);
+        total += count;
+    }
+};
+
+static BLaunch finish(TableBuf &tb, const ItemList &L) {
+    BLaunch b;
+    b.n_items = (int)L.items.size();
+    b.grid = L.total;
+    b.items_off = tb.add(L.items.data(), L.items.size());
+    return b;
+}
+
+struct MVPlan {
+    size_t reqs_off = 0;
+    BLaunch gather, red, leaf;
+    std::vector<BLaunch> fwd, blocks, bwd;
+};
+
+// matvec tables for handles hs with inputs x, outputs y, shifts
+static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const double *const *x,
+                          double *const *y, const double *shift) {
+    MVPlan P;
+    std::vector<ReqCtx> reqs(n);
+    size_t max_fwd = 0, max_bwd = 0;
+    for (int i = 0; i < n; i++) {
+        const h2p_hip_handle *h = hs[i];
+        ReqCtx &c = reqs[i];
+        c.coord = h->d_coord;
+        c.CT = h->CT;
+        c.scale = kscale(h->kid, h->kp.p0);
+        c.diag = h->kp.p0;
+        c.kp = h->kp;
+        c.xin = h->d_xin;
+        c.y1 = h->d_y1;
+        c.yp = h->d_yp;
+        c.arena = h->d_arena;
+        c.U = h->d_U;
+        c.UT = h->d_UT;
+        c.blob = h->d_blob;
+        c.fwd = h->d_fwd;
+        c.contrib = h->d_contrib;
+        c.N = h->N;
+        c.pad = 0;
+        c.x = x[i];
+        c.y = y[i];
+        c.shift = shift ? shift[i] : 0.0;
+        max_fwd = std::max(max_fwd, h->fwd_levels.size());
+        max_bwd = std::max(max_bwd, h->bwd_levels.size());
+    }
+    P.reqs_off = tb.add(reqs.data(), reqs.size());
+    ItemList g, r, lf;
+    for (int i = 0; i < n; i++) {
+        g.add(i, nullptr, (hs[i]->N + 255) / 256);
+        r.add(i, hs[i]->d_red, hs[i]->n_red);
+        lf.add(i, hs[i]->d_leaf, hs[i]->n_leaf);
+    }
+    P.gather = finish(tb, g);
+    P.red = finish(tb, r);
+    P.leaf = finish(tb, lf);
+    for (size_t k = 0; k < max_fwd; k++) {
+        ItemList L;
+        for (int i = 0; i < n; i++)
+            if (k < hs[i]->fwd_levels.size()) {
+                const Level &lv = hs[i]->fwd_levels[k];
+                L.add(i, hs[i]->d_fwdt + lv.off, lv.n);
+            }
+        P.fwd.push_back(finish(tb, L));
+    }
+    for (size_t k = 0; k < max_bwd; k++) {
+        ItemList L;
+        for (int i = 0; i < n; i++)
+            if (k < hs[i]->bwd_levels.size()) {
+                const Level &lv = hs[i]->bwd_levels[k];
+                L.add(i, hs[i]->d_bwdt + lv.off, lv.n);
+            }
+        P.bwd.push_back(finish(tb, L));
+    }
+    // block launches grouped by kernel choice, in order of first appearance
+    std::vector<int> done(n, 0);
+    for (int i = 0; i < n; i++) {
+        if (done[i]) continue;
+        ItemList L;
+        for (int j = i; j < n; j++) {
+            const h2p_hip_handle *a = hs[i], *b = hs[j];
+            if (done[j] || a->mode != b->mode || a->prec != b->prec || a->kid != b->kid) continue;
+            done[j] = 1;
+            L.add(j, b->d_blk, b->n_blk);
+        }
+        BLaunch bl = finish(tb, L);
+        bl.mode = hs[i]->mode;
+        bl.prec = hs[i]->prec;
+        bl.kid = hs[i]->kid;
+        if (bl.grid > 0) P.blocks.push_back(bl);
+    }
+    return P;
+}
+
+template <typename K>
+static void launch_items(K kernel, const BLaunch &b, const char *dbase, const void *reqs, hipStream_t s) {
+    if (b.grid <= 0) return;
+    hipLaunchKernelGGL(kernel, dim3(b.grid), dim3(256), 0, s, (const BatchItem *)(dbase + b.items_off), b.n_items,
+                       (const ReqCtx *)reqs);
+}
+
+static void launch_blocks_b(const BLaunch &b, const char *dbase, const ReqCtx *reqs, hipStream_t s) {
+    const BatchItem *it = (const BatchItem *)(dbase + b.items_off);
+    const dim3 grid(b.grid), blk(256);
+    const bool f32 = b.prec == H2P_HIP_FP32MIX;
+    if (b.mode == H2P_HIP_AOT) {
+        if (f32) hipLaunchKernelGGL(k_blocks_aot_b<float>, grid, blk, 0, s, it, b.n_items, reqs);
+        else hipLaunchKernelGGL(k_blocks_aot_b<double>, grid, blk, 0, s, it, b.n_items, reqs);
+        return;
+    }
+    switch (b.kid) {
+        case KC:
+            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KC, float>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KC, double>), grid, blk, 0, s, it, b.n_items, reqs);
+            break;
+        case KG:
+            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KG, float>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KG, double>), grid, blk, 0, s, it, b.n_items, reqs);
+            break;
+        case KM:
+            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KM, float>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KM, double>), grid, blk, 0, s, it, b.n_items, reqs);
+            break;
+        default:
+            hipLaunchKernelGGL((k_blocks_jit_b<KR, 3>), grid, blk, 0, s, it, b.n_items, reqs);
+            break;
+    }
+}
+
+static void launch_matvec_b(const MVPlan &P, const char *dbase, hipStream_t s) {
+    const ReqCtx *reqs = (const ReqCtx *)(dbase + P.reqs_off);
+    launch_items(k_gather_b, P.gather, dbase, reqs, s);
+    for (const BLaunch &b : P.fwd) launch_items(k_fwd2_b, b, dbase, reqs, s);
+    for (const BLaunch &b : P.blocks) launch_blocks_b(b, dbase, reqs, s);
+    launch_items(k_reduce_y1_b, P.red, dbase, reqs, s);
+    for (const BLaunch &b : P.bwd) launch_items(k_bwd2_b, b, dbase, reqs, s);
+    launch_items(k_leaf_final2_b, P.leaf, dbase, reqs, s);
+}
+
+struct CGPlan {
+    size_t cgs_off = 0, first_off = 0;
+    int n = 0, grid = 0;
+};
+
+static CGPlan plan_cg(TableBuf &tb, int n, h2p_hip_handle *const *hs) {
+    CGPlan P;
+    std::vector<CGCtx> cgs(n);
+    std::vector<int> first(n);
+    int total = 0;
+    for (int i = 0; i < n; i++) {
+        const h2p_hip_handle *h = hs[i];
+        CGCtx &c = cgs[i];
+        c.x = h->cg_x;
+        c.r = h->cg_r;
+        c.p = h->cg_p;
+        c.s = h->cg_s;
+        c.b = h->cg_b;
+        c.part = h->cg_part;
+        c.sc = h->cg_sc;
+        c.tol = h->cg_tol;
+        c.N = h->N;
+        c.nblk = h->cg_nblk;
+        c.max_iter = h->cg_max_iter;
+        c.pad = 0;
+        first[i] = total;
+        total += h->cg_nblk;
+    }
+    P.cgs_off = tb.add(cgs.data(), cgs.size());
+    P.first_off = tb.add(first.data(), first.size());
+    P.n = n;
+    P.grid = total;
+    return P;
+}
+
+}  // namespace
+
+struct h2p_hip_batch {
+    static constexpr int NSLOT = 2;
+    char *h_stage[NSLOT] = {nullptr, nullptr};
+    char *d_stage[NSLOT] = {nullptr, nullptr};
+    size_t cap[NSLOT] = {0, 0};
+    hipEvent_t ev[NSLOT] = {nullptr, nullptr};
+    bool ev_live[NSLOT] = {false, false};
+    int slot = 0;
+    CGScal *h_poll = nullptr;   // pinned read-back of solver scalars
+    int poll_cap = 0;
+};
+
+static void free_batch(h2p_hip_batch *b) {
+    if (!b) return;
+    for (int k = 0; k < h2p_hip_batch::NSLOT; k++) {
+        if (b->ev[k]) {
+            (void)hipEventSynchronize(b->ev[k]);
+            (void)hipEventDestroy(b->ev[k]);
+        }
+        if (b->h_stage[k]) (void)hipHostFree(b->h_stage[k]);
+        if (b->d_stage[k]) (void)hipFree(b->d_stage[k]);
+    }
+    if (b->h_poll) (void)hipHostFree(b->h_poll);
+    delete b;
+}
+
+// Copy the tables into the next slot and return the device base of the slot.
+static int stage_tables(h2p_hip_batch *B, const TableBuf &tb, hipStream_t s, const char **dbase) {
+    const int k = B->slot;
+    B->slot = (B->slot + 1) % h2p_hip_batch::NSLOT;
+    if (B->ev_live[k]) HIP_TRY(hipEventSynchronize(B->ev[k]));
+    const size_t need = std::max<size_t>(tb.bytes.size(), 256);
+    if (B->cap[k] < need) {
+        const size_t c = std::max(need, 2 * B->cap[k]);
+        if (B->h_stage[k]) (void)hipHostFree(B->h_stage[k]);
+        if (B->d_stage[k]) (void)hipFree(B->d_stage[k]);
+        B->h_stage[k] = nullptr;
+        B->d_stage[k] = nullptr;
+        B->cap[k] = 0;
+        if (hipHostMalloc((void **)&B->h_stage[k], c, hipHostMallocDefault) != hipSuccess)
+            return fail(H2P_HIP_ENOMEM, "hipHostMalloc of the batch staging failed");
+        if (hipMalloc((void **)&B->d_stage[k], c) != hipSuccess)
+            return fail(H2P_HIP_ENOMEM, "hipMalloc of the batch tables failed");
+        B->cap[k] = c;
+    }
+    std::memcpy(B->h_stage[k], tb.bytes.data(), tb.bytes.size());
+    HIP_TRY(hipMemcpyAsync(B->d_stage[k], B->h_stage[k], tb.bytes.size(), hipMemcpyHostToDevice, s));
+    *dbase = B->d_stage[k];
+    return 0;
+}
+
+static int release_slot(h2p_hip_batch *B, hipStream_t s) {
+    // the slot just used is the one before B->slot
+    const int k = (B->slot + h2p_hip_batch::NSLOT - 1) % h2p_hip_batch::NSLOT;
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(B->ev[k], s));
+    B->ev_live[k] = true;
+    return 0;
+}
+
+static int check_handles(int n, h2p_hip_handle *const *hs) {
+    if (n < 0) return fail(H2P_HIP_EINVAL, "n < 0");
+    if (n > 0 && !hs) return fail(H2P_HIP_EINVAL, "NULL handle array");
+    for (int i = 0; i < n; i++) {
+        if (!hs[i]) return fail(H2P_HIP_EINVAL, "NULL handle");
+        for (int j = 0; j < i; j++)
+            if (hs[j] == hs[i]) return fail(H2P_HIP_EINVAL, "a handle appears twice in one batch");
+    }
+    return 0;
+}
+
+extern "C" {
+
+int h2p_hip_batch_create(h2p_hip_batch **out) {
+    if (!out) return fail(H2P_HIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    h2p_hip_batch *b = new h2p_hip_batch();
+    for (int k = 0; k < h2p_hip_batch::NSLOT; k++)
+        if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) {
+            free_batch(b);
+            return fail(H2P_HIP_EDEVICE, "hipEventCreate failed");
+        }
+    *out = b;
+    return 0;
+}
+
+void h2p_hip_batch_free(h2p_hip_batch *b) { free_batch(b); }
+
+int h2p_hip_matvec_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, const double *const *d_x,
+                         double *const *d_y, const double *shift, void *stream) {
+    if (!B) return fail(H2P_HIP_EINVAL, "NULL batch");
+    int rc = check_handles(n, hs);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (!d_x || !d_y) return fail(H2P_HIP_EINVAL, "NULL vector array");
+    for (int i = 0; i < n; i++)
+        if (!d_x[i] || !d_y[i]) return fail(H2P_HIP_EINVAL, "NULL vector");
+    hipStream_t s = (hipStream_t)stream;
+    TableBuf tb;
+    const MVPlan P = plan_matvec(tb, n, hs, d_x, d_y, shift);
+    const char *dbase = nullptr;
+    if ((rc = stage_tables(B, tb, s, &dbase))) return rc;
+    launch_matvec_b(P, dbase, s);
+    return release_slot(B, s);
+}
+
+int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int layout, const int *n_vec,
+                         const double *const *d_X, const int *ldx, double *const *d_Y, const int *ldy,
+                         void *stream) {
+    if (!B) return fail(H2P_HIP_EINVAL, "NULL batch");
+    int rc = check_handles(n, hs);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (!n_vec || !d_X || !ldx || !d_Y || !ldy) return fail(H2P_HIP_EINVAL, "NULL argument array");
+    const bool rm = layout == H2P_HIP_ROW_MAJOR;
+    if (!rm && layout != H2P_HIP_COL_MAJOR) return fail(H2P_HIP_EINVAL, "bad layout");
+    int max_nv = 0;
+    for (int i = 0; i < n; i++) {
+        const h2p_hip_handle *h = hs[i];
+        if (n_vec[i] < 0) return fail(H2P_HIP_EINVAL, "n_vec < 0");
+        if (n_vec[i] > 0 && (!d_X[i] || !d_Y[i])) return fail(H2P_HIP_EINVAL, "NULL panel");
+        if (n_vec[i] > 0 && (rm ? (ldx[i] < n_vec[i] || ldy[i] < n_vec[i]) : (ldx[i] < h->N || ldy[i] < h->N)))
+            return fail(H2P_HIP_EINVAL, "bad leading dimension");
+        if (n_vec[i] > 0 && !h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
+        max_nv = std::max(max_nv, n_vec[i]);
+    }
+    for (int i = 0; i < n; i++)
+        if (n_vec[i] > 0 && (rc = ensure_panels(hs[i], panel_width(n_vec[i])))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    for (int v0 = 0; v0 < max_nv; v0 += H2P_MM_MAXNVP) {
+        // requests with columns left in this chunk
+        std::vector<h2p_hip_handle *> act;
+        std::vector<MMCtx> ctx;
+        for (int i = 0; i < n; i++) {
+            if (n_vec[i] <= v0) continue;
+            h2p_hip_handle *h = hs[i];
+            MMCtx c{};
+            c.nv = std::min(H2P_MM_MAXNVP, n_vec[i] - v0);
+            c.srcs = h->d_mms;
+            c.U = h->d_U;
+            c.UT = h->d_UT;
+            c.BD = (const double *)h->d_blob;
+            c.coord = h->d_coord;
+            c.CT = h->CT;
+            c.kp = h->kp;
+            c.P = panels_of(h, (c.nv + 15) & ~15);
+            c.fwd = h->d_fwd;
+            c.N = h->N;
+            c.X = d_X[i];
+            c.Y = d_Y[i];
+            c.ldx = ldx[i];
+            c.ldy = ldy[i];
+            c.row_major = rm ? 1 : 0;
+            c.v0 = v0;
+            act.push_back(h);
+            ctx.push_back(c);
+        }
+        const int m = (int)act.size();
+        TableBuf tb;
+        const size_t ctx_off = tb.add(ctx.data(), ctx.size());
+        ItemList g, sc;
+        size_t max_fwd = 0, max_bwd = 0;
+        for (int i = 0; i < m; i++) {
+            const long long ne = (long long)act[i]->N * ctx[i].P.nvp, no = (long long)act[i]->N * ctx[i].nv;
+            g.add(i, nullptr, (int)((ne + 255) / 256));
+            sc.add(i, nullptr, (int)((no + 255) / 256));
+            max_fwd = std::max(max_fwd, act[i]->mm_fwd_levels.size());
+            max_bwd = std::max(max_bwd, act[i]->mm_bwd_levels.size());
+        }
+        const BLaunch lg = finish(tb, g), ls = finish(tb, sc);
+        // product phases, grouped by kernel id: (level list, launch)
+        struct MMLaunch { BLaunch b; int kid; };
+        std::vector<MMLaunch> phases;
+        auto add_phase = [&](auto level_of) {
+            std::vector<int> used(m, 0);
+            for (int i = 0; i < m; i++) {
+                if (used[i]) continue;
+                ItemList L;
+                for (int j = i; j < m; j++) {
+                    if (used[j] || act[j]->kid != act[i]->kid) continue;
+                    used[j] = 1;
+                    const Level *lv = level_of(act[j]);
+                    if (lv) L.add(j, act[j]->d_mmt + lv->off, lv->n);
+                }
+                BLaunch b = finish(tb, L);
+                if (b.grid > 0) phases.push_back({b, act[i]->kid});
+            }
+        };
+        for (size_t k = 0; k < max_fwd; k++)
+            add_phase([&](h2p_hip_handle *h) { return k < h->mm_fwd_levels.size() ? &h->mm_fwd_levels[k] : (const Level *)nullptr; });
+        add_phase([&](h2p_hip_handle *h) { return (const Level *)&h->mm_intmd; });
+        for (size_t k = 0; k < max_bwd; k++)
+            add_phase([&](h2p_hip_handle *h) { return k < h->mm_bwd_levels.size() ? &h->mm_bwd_levels[k] : (const Level *)nullptr; });
+        add_phase([&](h2p_hip_handle *h) { return (const Level *)&h->mm_leaf; });
+        const char *dbase = nullptr;
+        if ((rc = stage_tables(B, tb, s, &dbase))) return rc;
+        const MMCtx *dctx = (const MMCtx *)(dbase + ctx_off);
+        if (lg.grid > 0)
+            hipLaunchKernelGGL(k_mm_gather_b, dim3(lg.grid), dim3(256), 0, s, (const BatchItem *)(dbase + lg.items_off),
+                               lg.n_items, dctx);
+        for (const MMLaunch &ph : phases) {
+            const BatchItem *it = (const BatchItem *)(dbase + ph.b.items_off);
+            const dim3 grid(ph.b.grid), blk(256);
+            switch (ph.kid) {
+                case KC: hipLaunchKernelGGL((k_mm_b<KC, 1>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KG: hipLaunchKernelGGL((k_mm_b<KG, 1>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KM: hipLaunchKernelGGL((k_mm_b<KM, 1>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                default: hipLaunchKernelGGL((k_mm_b<KR, 3>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+            }
+        }
+        if (ls.grid > 0)
+            hipLaunchKernelGGL(k_mm_scatter_b, dim3(ls.grid), dim3(256), 0, s, (const BatchItem *)(dbase + ls.items_off),
+                               ls.n_items, dctx);
+        if ((rc = release_slot(B, s))) return rc;
+    }
+    return 0;
+}
+
+int h2p_hip_cg_start(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, const double *const *d_b,
+                     double *const *d_x, const double *shift, const double *tol, const int *max_iter,
+                     void *stream) {
+    if (!B) return fail(H2P_HIP_EINVAL, "NULL batch");
+    int rc = check_handles(n, hs);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (!d_b || !d_x || !tol || !max_iter) return fail(H2P_HIP_EINVAL, "NULL argument array");
+    for (int i = 0; i < n; i++) {
+        if (!d_b[i] || !d_x[i]) return fail(H2P_HIP_EINVAL, "NULL vector");
+        if (!(tol[i] >= 0.0)) return fail(H2P_HIP_EINVAL, "tol must be >= 0");
+    }
+    for (int i = 0; i < n; i++) {
+        h2p_hip_handle *h = hs[i];
+        if (!h->d_cg) {
+            const size_t nd = cg_doubles(h->N);
+            if (hipMalloc(&h->d_cg, sizeof(double) * nd) != hipSuccess)
+                return fail(H2P_HIP_ENOMEM, "hipMalloc of the CG state failed");
+            h->device_bytes += sizeof(double) * nd;
+            h->info.device_bytes = h->device_bytes;
+            h->cg_nblk = cg_blocks(h->N);
+            h->cg_r = h->d_cg;
+            h->cg_p = h->cg_r + h->N;
+            h->cg_s = h->cg_p + h->N;
+            h->cg_part = h->cg_s + h->N;
+            h->cg_sc = (CGScal *)(h->cg_part + 2 * (size_t)h->cg_nblk);
+        }
+        h->cg_x = d_x[i];
+        h->cg_b = d_b[i];
+        h->cg_shift = shift ? shift[i] : 0.0;
+        h->cg_tol = tol[i];
+        h->cg_max_iter = max_iter[i];
+        h->cg_started = 1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    TableBuf tb;
+    std::vector<const double *> xs(n);
+    std::vector<double *> ys(n);
+    std::vector<double> sh(n);
+    for (int i = 0; i < n; i++) {
+        xs[i] = hs[i]->cg_x;
+        ys[i] = hs[i]->cg_s;
+        sh[i] = hs[i]->cg_shift;
+    }
+    const MVPlan P = plan_matvec(tb, n, hs, xs.data(), ys.data(), sh.data());
+    const CGPlan C = plan_cg(tb, n, hs);
+    const char *dbase = nullptr;
+    if ((rc = stage_tables(B, tb, s, &dbase))) return rc;
+    const int *first = (const int *)(dbase + C.first_off);
+    const CGCtx *cgs = (const CGCtx *)(dbase + C.cgs_off);
+    launch_matvec_b(P, dbase, s);
+    hipLaunchKernelGGL(k_cg_init, dim3(C.grid), dim3(256), 0, s, first, C.n, cgs);
+    hipLaunchKernelGGL(k_cg_init_finish, dim3(C.n), dim3(64), 0, s, C.n, cgs);
+    return release_slot(B, s);
+}
+
+int h2p_hip_cg_step(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int n_iter, void *stream) {
+    if (!B) return fail(H2P_HIP_EINVAL, "NULL batch");
+    int rc = check_handles(n, hs);
+    if (rc) return rc;
+    if (n_iter < 0) return fail(H2P_HIP_EINVAL, "n_iter < 0");
+    if (n == 0 || n_iter == 0) return 0;
+    for (int i = 0; i < n; i++)
+        if (!hs[i]->cg_started) return fail(H2P_HIP_EINVAL, "CG step on a handle without h2p_hip_cg_start");
+    hipStream_t s = (hipStream_t)stream;
+    TableBuf tb;
+    std::vector<const double *> xs(n);
+    std::vector<double *> ys(n);
+    std::vector<double> sh(n);
+    for (int i = 0; i < n; i++) {
+        xs[i] = hs[i]->cg_p;
+        ys[i] = hs[i]->cg_s;
+        sh[i] = hs[i]->cg_shift;
+    }
+    const MVPlan P = plan_matvec(tb, n, hs, xs.data(), ys.data(), sh.data());
+    const CGPlan C = plan_cg(tb, n, hs);
+    const char *dbase = nullptr;
+    if ((rc = stage_tables(B, tb, s, &dbase))) return rc;
+    const int *first = (const int *)(dbase + C.first_off);
+    const CGCtx *cgs = (const CGCtx *)(dbase + C.cgs_off);
+    for (int it = 0; it < n_iter; it++) {
+        hipLaunchKernelGGL(k_cg_p, dim3(C.grid), dim3(256), 0, s, first, C.n, cgs);
+        launch_matvec_b(P, dbase, s);
+        hipLaunchKernelGGL(k_cg_dot_ps, dim3(C.grid), dim3(256), 0, s, first, C.n, cgs);
+        hipLaunchKernelGGL(k_cg_alpha, dim3(C.n), dim3(64), 0, s, C.n, cgs);
+        hipLaunchKernelGGL(k_cg_update, dim3(C.grid), dim3(256), 0, s, first, C.n, cgs);
+        hipLaunchKernelGGL(k_cg_finish, dim3(C.n), dim3(64), 0, s, C.n, cgs);
+    }
+    return release_slot(B, s);
+}
+
+int h2p_hip_cg_poll(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int *done, int *iter, double *relres,
+                    void *stream) {
+    if (!B) return fail(H2P_HIP_EINVAL, "NULL batch");
+    int rc = check_handles(n, hs);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    for (int i = 0; i < n; i++)
+        if (!hs[i]->cg_started) return fail(H2P_HIP_EINVAL, "CG poll on a handle without h2p_hip_cg_start");
+    hipStream_t s = (hipStream_t)stream;
+    if (B->poll_cap < n) {
+        if (B->h_poll) (void)hipHostFree(B->h_poll);
+        B->h_poll = nullptr;
+        B->poll_cap = 0;
+        if (hipHostMalloc((void **)&B->h_poll, sizeof(CGScal) * n, hipHostMallocDefault) != hipSuccess)
+            return fail(H2P_HIP_ENOMEM, "hipHostMalloc failed");
+        B->poll_cap = n;
+    }
+    for (int i = 0; i < n; i++)
+        HIP_TRY(hipMemcpyAsync(B->h_poll + i, hs[i]->cg_sc, sizeof(CGScal), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) {
+        const CGScal &c = B->h_poll[i];
+        if (done) done[i] = c.done;
+        if (iter) iter[i] = c.iter;
+        if (relres) relres[i] = std::sqrt(c.rr) / std::sqrt(c.bb);   // 0 / 0 = NaN for b = 0, as pcg.c:126
+    }
+    return 0;
 }
 
 }  // extern "C"

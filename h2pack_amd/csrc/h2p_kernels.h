@@ -30,21 +30,9 @@ struct FwdTask {
     int nrow, r, in_off, out_off;
 };
 
-// Downward-sweep task: y1[out .. out+nrow) += U y1[in .. in+r), U^T stored row-major.
-struct BwdTask {
-    long long ut_off;
-    int nrow, r, in_off, out_off;
-};
-
 // Segmented reduction into y1: y1[dst + e] = sum_k arena[contrib[k] + e].
 struct RedTask {
     int dst, len, cbeg, cend;
-};
-
-// Leaf output: y[fwd[vec_s + i]] = (U y1)[i] + sum_k arena[contrib[k] + i].
-struct LeafTask {
-    long long ut_off;
-    int nrow, r, y1_off, vec_s, cbeg, cend, active, pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -54,62 +42,9 @@ __global__ void k_gather(const double *__restrict__ x, const int *__restrict__ i
     if (i < n) out[i] = x[idx[i]];
 }
 
-// Upward sweep for one level, src/H2Pack_matvec.c:95-177 (one workgroup per
-// node; lanes own U columns, the four waves split the rows).
-__global__ void __launch_bounds__(256) k_fwd(const FwdTask *__restrict__ tasks,
-                                             const double *__restrict__ U,
-                                             double *__restrict__ xin) {
-    __shared__ double red[4][H2P_WAVE];
-    const FwdTask t = tasks[blockIdx.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double *Ut = U + t.u_off;
-    const double *in = xin + t.in_off;
-    for (int cs = 0; cs < t.r; cs += H2P_WAVE) {
-        const int c = cs + lane;
-        double acc = 0.0;
-        if (c < t.r) {
-            int i = w;
-            for (; i + 12 < t.nrow; i += 16) {
-                double u0 = Ut[(long long)i * t.r + c];
-                double u1 = Ut[(long long)(i + 4) * t.r + c];
-                double u2 = Ut[(long long)(i + 8) * t.r + c];
-                double u3 = Ut[(long long)(i + 12) * t.r + c];
-                acc += u0 * in[i];
-                acc += u1 * in[i + 4];
-                acc += u2 * in[i + 8];
-                acc += u3 * in[i + 12];
-            }
-            for (; i < t.nrow; i += 4) acc += Ut[(long long)i * t.r + c] * in[i];
-        }
-        red[w][lane] = acc;
-        __syncthreads();
-        if (w == 0 && c < t.r)
-            xin[t.out_off + c] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-        __syncthreads();
-    }
-}
-
-// Downward sweep for one level, src/H2Pack_matvec.c:767-861 (push-down into
-// the children's contiguous y1 slice; one writer per child).
-__global__ void __launch_bounds__(256) k_bwd(const BwdTask *__restrict__ tasks,
-                                             const double *__restrict__ UT,
-                                             double *__restrict__ y1) {
-    const BwdTask t = tasks[blockIdx.x];
-    const double *ut = UT + t.ut_off;
-    const double *in = y1 + t.in_off;
-    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) {
-        double acc = 0.0;
-        for (int c = 0; c < t.r; c++) acc += ut[(long long)c * t.nrow + i] * in[c];
-        y1[t.out_off + i] += acc;
-    }
-}
-
 // Fixed-order segmented reduction of the block partials into y1.
-__global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ tasks,
-                                                   const long long *__restrict__ contrib,
-                                                   const double *__restrict__ arena,
-                                                   double *__restrict__ y1) {
-    const RedTask t = tasks[blockIdx.x];
+__device__ __forceinline__ void reduce_y1_body(const RedTask &t, const long long *__restrict__ contrib,
+                                               const double *__restrict__ arena, double *__restrict__ y1) {
     for (int e = threadIdx.x; e < t.len; e += blockDim.x) {
         double s = 0.0;
         for (int k = t.cbeg; k < t.cend; k++) s += arena[contrib[k] + e];
@@ -117,40 +52,23 @@ __global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ t
     }
 }
 
-// Leaf outputs: leaf part of the downward sweep plus every block partial that
-// lands on the leaf, written straight to the caller's order (y[fwd[i]] = py[i]
-// is the backward permutation of src/H2Pack_matvec.c:1156-1159).
-__global__ void __launch_bounds__(256) k_leaf_final(const LeafTask *__restrict__ tasks,
-                                                    const long long *__restrict__ contrib,
-                                                    const double *__restrict__ arena,
-                                                    const double *__restrict__ UT,
-                                                    const double *__restrict__ y1,
-                                                    const int *__restrict__ fwd,
-                                                    double *__restrict__ y) {
-    const LeafTask t = tasks[blockIdx.x];
-    const double *ut = UT + t.ut_off;
-    const double *yv = y1 + t.y1_off;
-    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) {
-        double v = 0.0;
-        if (t.active)
-            for (int c = 0; c < t.r; c++) v += ut[(long long)c * t.nrow + i] * yv[c];
-        for (int k = t.cbeg; k < t.cend; k++) v += arena[contrib[k] + i];
-        y[fwd[t.vec_s + i]] = v;
-    }
+__global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ tasks,
+                                                   const long long *__restrict__ contrib,
+                                                   const double *__restrict__ arena,
+                                                   double *__restrict__ y1) {
+    reduce_y1_body(tasks[blockIdx.x], contrib, arena, y1);
 }
 
 // ---------------------------------------------------------------------------
-// Sweep kernels, second form.
+// Sweep kernels.
 //
-// Upward sweep: one workgroup per node; lanes own up to 256 columns of U at
+// Upward sweep (src/H2Pack_matvec.c:95-177): one workgroup per node; lanes own up to 256 columns of U at
 // once (4 strips), the four waves take contiguous quarters of the rows and
 // keep 4 rows x 4 strips of loads in flight; the four wave partials are
 // added in LDS in a fixed order.
-__global__ void __launch_bounds__(256) k_fwd2(const FwdTask *__restrict__ tasks,
-                                              const double *__restrict__ U,
-                                              double *__restrict__ xin) {
+__device__ __forceinline__ void fwd2_body(const FwdTask &t, const double *__restrict__ U,
+                                          double *__restrict__ xin) {
     __shared__ double red[4][4][H2P_WAVE];
-    const FwdTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *Ut = U + t.u_off;
     const double *in = xin + t.in_off;
@@ -198,7 +116,13 @@ __global__ void __launch_bounds__(256) k_fwd2(const FwdTask *__restrict__ tasks,
     }
 }
 
-// Downward sweep, one workgroup per 256-row chunk of a node:
+__global__ void __launch_bounds__(256) k_fwd2(const FwdTask *__restrict__ tasks,
+                                              const double *__restrict__ U,
+                                              double *__restrict__ xin) {
+    fwd2_body(tasks[blockIdx.x], U, xin);
+}
+
+// Downward sweep (src/H2Pack_matvec.c:767-861), one workgroup per 256-row chunk of a node:
 // yp[child rows] = U (y1 + yp)[node], U^T stored row-major.  The push-down
 // buffer yp is written exactly once per row (each child has one parent).
 struct BwdTask2 {
@@ -206,12 +130,9 @@ struct BwdTask2 {
     int nrow, r, y1_off, push, out_off, row0;
 };
 #define H2P_MAXR 2048
-__global__ void __launch_bounds__(256) k_bwd2(const BwdTask2 *__restrict__ tasks,
-                                              const double *__restrict__ UT,
-                                              const double *__restrict__ y1,
-                                              double *__restrict__ yp) {
+__device__ __forceinline__ void bwd2_body(const BwdTask2 &t, const double *__restrict__ UT,
+                                          const double *__restrict__ y1, double *__restrict__ yp) {
     __shared__ double yv[H2P_MAXR];
-    const BwdTask2 t = tasks[blockIdx.x];
     for (int c = threadIdx.x; c < t.r; c += blockDim.x)
         yv[c] = y1[t.y1_off + c] + (t.push ? yp[t.y1_off + c] : 0.0);
     __syncthreads();
@@ -230,7 +151,15 @@ __global__ void __launch_bounds__(256) k_bwd2(const BwdTask2 *__restrict__ tasks
     yp[t.out_off + row] = (a0 + a1) + (a2 + a3);
 }
 
-// Leaf outputs, second form: the leaf's skeleton value is its own block
+__global__ void __launch_bounds__(256) k_bwd2(const BwdTask2 *__restrict__ tasks,
+                                              const double *__restrict__ UT,
+                                              const double *__restrict__ y1,
+                                              double *__restrict__ yp) {
+    bwd2_body(tasks[blockIdx.x], UT, y1, yp);
+}
+
+// Leaf outputs (src/H2Pack_matvec.c:767-861 for the leaves, and the backward
+// permutation y[fwd[i]] = py[i] of src/H2Pack_matvec.c:1156-1159): the leaf's skeleton value is its own block
 // partials plus its parent's push-down; then U y1 plus the point partials
 // go straight to the caller's order.
 struct LeafTask2 {
@@ -240,15 +169,16 @@ struct LeafTask2 {
     int cb0, ce0;   // contributions in point space (length nrow)
     int pad;
 };
-__global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict__ tasks,
-                                                     const long long *__restrict__ contrib,
-                                                     const double *__restrict__ arena,
-                                                     const double *__restrict__ UT,
-                                                     const double *__restrict__ yp,
-                                                     const int *__restrict__ fwd,
-                                                     double *__restrict__ y) {
+// shift != 0 adds shift * x (a diagonal nugget applied outside the H2
+// product, as the PCG wrapper of examples/SPDHSS-H2/pcg_tests.c:15-21 does);
+// px holds x in the permuted order.
+__device__ __forceinline__ void leaf_final2_body(const LeafTask2 &t, const long long *__restrict__ contrib,
+                                                 const double *__restrict__ arena,
+                                                 const double *__restrict__ UT,
+                                                 const double *__restrict__ yp,
+                                                 const int *__restrict__ fwd, double *__restrict__ y,
+                                                 const double *__restrict__ px, double shift) {
     __shared__ double yv[H2P_MAXR];
-    const LeafTask2 t = tasks[blockIdx.x];
     for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
         double v = t.push ? yp[t.s_off + c] : 0.0;
         for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
@@ -268,8 +198,20 @@ __global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict
         for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
         double v = (a0 + a1) + (a2 + a3);
         for (int k = t.cb0; k < t.ce0; k++) v += arena[contrib[k] + i];
+        if (shift != 0.0) v += shift * px[t.vec_s + i];
         y[fwd[t.vec_s + i]] = v;
     }
+}
+
+__global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict__ tasks,
+                                                     const long long *__restrict__ contrib,
+                                                     const double *__restrict__ arena,
+                                                     const double *__restrict__ UT,
+                                                     const double *__restrict__ yp,
+                                                     const int *__restrict__ fwd,
+                                                     double *__restrict__ y,
+                                                     const double *__restrict__ px, double shift) {
+    leaf_final2_body(tasks[blockIdx.x], contrib, arena, UT, yp, fwd, y, px, shift);
 }
 
 // ---------------------------------------------------------------------------
@@ -359,12 +301,10 @@ __device__ __forceinline__ void blk_aot_pass(const BlkTask &t, int cbase, int nc
 
 #define H2P_AOT_MAXNS 4
 template <typename TS>
-__global__ void __launch_bounds__(256) k_blocks_aot(const BlkTask *__restrict__ tasks,
-                                                    const TS *__restrict__ blob,
-                                                    const double *__restrict__ xin,
-                                                    double *__restrict__ arena) {
+__device__ __forceinline__ void blocks_aot_body(const BlkTask &t, const TS *__restrict__ blob,
+                                                const double *__restrict__ xin,
+                                                double *__restrict__ arena) {
     __shared__ double red[H2P_AOT_MAXNS][4][H2P_WAVE];
-    const BlkTask t = tasks[blockIdx.x];
     const TS *A = blob + t.data_off;
     for (int cb = 0; cb < t.ncol; cb += H2P_AOT_MAXNS * H2P_WAVE) {
         const int nc = min(t.ncol - cb, H2P_AOT_MAXNS * H2P_WAVE);
@@ -377,6 +317,14 @@ __global__ void __launch_bounds__(256) k_blocks_aot(const BlkTask *__restrict__ 
         // the next column pass adds into out0 rows written by the same lanes
         __syncthreads();
     }
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256) k_blocks_aot(const BlkTask *__restrict__ tasks,
+                                                    const TS *__restrict__ blob,
+                                                    const double *__restrict__ xin,
+                                                    double *__restrict__ arena) {
+    blocks_aot_body<TS>(tasks[blockIdx.x], blob, xin, arena);
 }
 
 __global__ void k_f64_to_f32(const double *__restrict__ in, float *__restrict__ out, long long n) {
@@ -478,13 +426,11 @@ __device__ __forceinline__ void blk_jit_pass(const BlkTask &t, int pbase, int np
 
 #define H2P_JIT_MAXNS 4
 template <int KID, int KD>
-__global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ tasks,
-                                                    const double *__restrict__ cx,
-                                                    long long cstride, KParam kp,
-                                                    const double *__restrict__ xin,
-                                                    double *__restrict__ arena) {
+__device__ __forceinline__ void blocks_jit_body(const BlkTask &t, const double *__restrict__ cx,
+                                                long long cstride, const KParam &kp,
+                                                const double *__restrict__ xin,
+                                                double *__restrict__ arena) {
     __shared__ double red[H2P_JIT_MAXNS][KD][4][H2P_WAVE];
-    const BlkTask t = tasks[blockIdx.x];
     for (int pb = 0; pb < t.n1; pb += H2P_JIT_MAXNS * H2P_WAVE) {
         const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
         const int ns = (np + 63) >> 6;
@@ -494,6 +440,15 @@ __global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ 
         else blk_jit_pass<KID, KD, 4>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
         __syncthreads();
     }
+}
+
+template <int KID, int KD>
+__global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ tasks,
+                                                    const double *__restrict__ cx,
+                                                    long long cstride, KParam kp,
+                                                    const double *__restrict__ xin,
+                                                    double *__restrict__ arena) {
+    blocks_jit_body<KID, KD>(tasks[blockIdx.x], cx, cstride, kp, xin, arena);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,14 +597,12 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
 }
 
 template <int KID, typename T>
-__global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks,
-                                                     const double *__restrict__ cx,
-                                                     long long cstride, double scale, double diag,
-                                                     const double *__restrict__ xin,
-                                                     double *__restrict__ arena) {
+__device__ __forceinline__ void blocks_jit2_body(const BlkTask &t, const double *__restrict__ cx,
+                                                 long long cstride, double scale, double diag,
+                                                 const double *__restrict__ xin,
+                                                 double *__restrict__ arena) {
     __shared__ JitRow<T> rows[H2P_JIT_ROWS];
     __shared__ double red[H2P_JIT_MAXNS][4][H2P_WAVE];
-    const BlkTask t = tasks[blockIdx.x];
     double ox = 0.0, oy = 0.0, oz = 0.0;
     if constexpr (sizeof(T) == 4 || JitNorm<KID, T>::value) {   // block-local origin
         ox = cx[t.c0];
@@ -666,6 +619,15 @@ __global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__
         else jit2_pass<KID, T, 4>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
         __syncthreads();
     }
+}
+
+template <int KID, typename T>
+__global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks,
+                                                     const double *__restrict__ cx,
+                                                     long long cstride, double scale, double diag,
+                                                     const double *__restrict__ xin,
+                                                     double *__restrict__ arena) {
+    blocks_jit2_body<KID, T>(tasks[blockIdx.x], cx, cstride, scale, diag, xin, arena);
 }
 
 // ---------------------------------------------------------------------------

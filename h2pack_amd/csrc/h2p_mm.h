@@ -55,18 +55,16 @@ template <> struct MMGeom<3> { static constexpr int RC = 48, KC = 48; };
 #define H2P_MM_MAXNVP 64
 
 template <int KID, int KD>
-__global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
-                                            const MMSrc *__restrict__ srcs,
-                                            const double *__restrict__ Ublob,
-                                            const double *__restrict__ UTblob,
-                                            const double *__restrict__ BDblob,
-                                            const double *__restrict__ cx, long long cstride,
-                                            KParam kp, MMPanels P) {
+__device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict__ srcs,
+                                        const double *__restrict__ Ublob,
+                                        const double *__restrict__ UTblob,
+                                        const double *__restrict__ BDblob,
+                                        const double *__restrict__ cx, long long cstride,
+                                        const KParam &kp, const MMPanels &P) {
     constexpr int RC = MMGeom<KD>::RC, KC = MMGeom<KD>::KC;
     constexpr int MT = RC / 16;                 // M tiles per chunk (4 or 3)
     __shared__ double As[RC][KC + 1];
     __shared__ double Xs[KC][H2P_MM_MAXNVP + 1];
-    const MMTask t = tasks[blockIdx.x];
     const int nvp = P.nvp, NT = nvp >> 4;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
@@ -155,6 +153,17 @@ __global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
             }
         }
     }
+}
+
+template <int KID, int KD>
+__global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
+                                            const MMSrc *__restrict__ srcs,
+                                            const double *__restrict__ Ublob,
+                                            const double *__restrict__ UTblob,
+                                            const double *__restrict__ BDblob,
+                                            const double *__restrict__ cx, long long cstride,
+                                            KParam kp, MMPanels P) {
+    mm_body<KID, KD>(tasks[blockIdx.x], srcs, Ublob, UTblob, BDblob, cx, cstride, kp, P);
 }
 
 // Caller panel (original order, row- or column-major, ld) -> PX (permuted,

@@ -176,3 +176,90 @@ def direct_nbody(krnl_id: int, param, coord: np.ndarray, x: np.ndarray,
                                     coord.shape[1], coord.shape[1], x.ctypes.data,
                                     dst.ctypes.data, rows.size, rows.size, y.ctypes.data))
     return y
+
+
+def footprint(h2, mode: str = "jit", prec: str = "fp64", n_vec: int = 0,
+              with_cg: bool = False) -> int:
+    """Device bytes an upload of ``h2`` would hold (plus matmul panels for
+    ``n_vec`` columns and the CG state); allocates nothing on the device."""
+    lib = hip_lib()
+    m = {"aot": abi.AOT, "jit": abi.JIT}[mode]
+    p = {"fp64": abi.FP64, "fp32mix": abi.FP32MIX}[prec]
+    desc, keep = describe(h2, with_blocks=False)
+    kd = kernel_desc(h2.krnl_id, h2.krnl_param)
+    out = ct.c_size_t(0)
+    _check(lib.h2p_hip_footprint(ct.byref(desc), ct.byref(kd), m, p, int(n_vec),
+                                 1 if with_cg else 0, ct.byref(out)))
+    del keep
+    return int(out.value)
+
+
+def _ptrs(vals):
+    return (ct.c_void_p * max(len(vals), 1))(*[ct.c_void_p(int(v)) for v in vals])
+
+
+class H2Batch:
+    """Co-scheduled calls over several resident matrices (``h2p_hip_batch``).
+
+    Each method takes lists of ``H2MatHIP`` objects and device addresses (for
+    instance ``tensor.data_ptr()``) and enqueues on ``stream``; a matrix may
+    appear once per call.  The batched solver follows the reference's
+    unpreconditioned PCG (examples/PCG/pcg.c:11-152) on (A + shift I)."""
+
+    def __init__(self):
+        self.lib = hip_lib()
+        b = ct.c_void_p()
+        _check(self.lib.h2p_hip_batch_create(ct.byref(b)))
+        self.handle = b
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.h2p_hip_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _hs(mats):
+        return _ptrs([m.handle.value for m in mats])
+
+    def matvec(self, mats, x_ptrs, y_ptrs, shifts=None, stream: int = 0):
+        n = len(mats)
+        sh = None
+        if shifts is not None:
+            sh = (ct.c_double * max(n, 1))(*[float(v) for v in shifts])
+        _check(self.lib.h2p_hip_matvec_batch(self.handle, n, self._hs(mats), _ptrs(x_ptrs),
+                                             _ptrs(y_ptrs), sh, ct.c_void_p(stream)))
+
+    def matmul(self, mats, layout: int, n_vecs, x_ptrs, ldxs, y_ptrs, ldys, stream: int = 0):
+        n = len(mats)
+        ia = lambda v: (ct.c_int * max(n, 1))(*[int(a) for a in v])   # noqa: E731
+        _check(self.lib.h2p_hip_matmul_batch(self.handle, n, self._hs(mats), layout, ia(n_vecs),
+                                             _ptrs(x_ptrs), ia(ldxs), _ptrs(y_ptrs), ia(ldys),
+                                             ct.c_void_p(stream)))
+
+    def cg_start(self, mats, b_ptrs, x_ptrs, shifts, tols, max_iters, stream: int = 0):
+        n = len(mats)
+        da = lambda v: (ct.c_double * max(n, 1))(*[float(a) for a in v])   # noqa: E731
+        ia = (ct.c_int * max(n, 1))(*[int(a) for a in max_iters])
+        _check(self.lib.h2p_hip_cg_start(self.handle, n, self._hs(mats), _ptrs(b_ptrs),
+                                         _ptrs(x_ptrs), da(shifts), da(tols), ia,
+                                         ct.c_void_p(stream)))
+
+    def cg_step(self, mats, n_iter: int, stream: int = 0):
+        _check(self.lib.h2p_hip_cg_step(self.handle, len(mats), self._hs(mats), int(n_iter),
+                                        ct.c_void_p(stream)))
+
+    def cg_poll(self, mats, stream: int = 0):
+        """Waits for ``stream``; returns (done, iters, relres) lists."""
+        n = len(mats)
+        done = (ct.c_int * max(n, 1))()
+        it = (ct.c_int * max(n, 1))()
+        rr = (ct.c_double * max(n, 1))()
+        _check(self.lib.h2p_hip_cg_poll(self.handle, n, self._hs(mats), done, it, rr,
+                                        ct.c_void_p(stream)))
+        return [bool(done[i]) for i in range(n)], [it[i] for i in range(n)], [rr[i] for i in range(n)]

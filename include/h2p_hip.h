@@ -161,6 +161,63 @@ int h2p_hip_direct_nbody(const h2p_kernel_desc *krnl, int xpt_dim,
                          const double *x,
                          const double *dst, int dst_ld, int n_dst, double *y);
 
+/* ---- Co-scheduled execution of many resident matrices (serving) --------
+ *
+ * These calls batch what a reference program does one matrix at a time:
+ * H2P_matvec under the PCG shift wrapper (examples/SPDHSS-H2/pcg_tests.c:15-21,
+ * y = A x + shift x), the unpreconditioned pcg() of examples/PCG/pcg.c:11-152,
+ * and H2P_matmul.  One launch per phase serves every handle of the batch (a
+ * workgroup finds its handle and task through a small table), so blocks of
+ * different matrices run side by side.  Each handle keeps its own scratch and
+ * device-side solver state: a handle may appear once per call, and must not
+ * be used by another call (batched or not) that overlaps with it.  Results
+ * do not depend on which other handles share the batch.
+ *
+ * A batch context holds pinned staging for the launch tables; one context
+ * serves one thread, and its calls may go to different streams. */
+typedef struct h2p_hip_batch h2p_hip_batch;
+
+/* Device bytes h2p_hip_upload would hold for this representation, plus the
+ * matmul panels for n_vec columns (0: none) and the CG state (with_cg != 0).
+ * Allocates nothing on the device. */
+int h2p_hip_footprint(const h2p_hip_h2pack *h2, const h2p_kernel_desc *krnl,
+                      int mode, int prec, int n_vec, int with_cg, size_t *bytes);
+
+int h2p_hip_batch_create(h2p_hip_batch **out);
+void h2p_hip_batch_free(h2p_hip_batch *b);
+
+/* y_i := A_i x_i + shift_i x_i for i < n (device pointers, caller order;
+ * shift may be NULL for zeros). */
+int h2p_hip_matvec_batch(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
+                         const double *const *d_x, double *const *d_y,
+                         const double *shift, void *stream);
+
+/* Y_i := A_i X_i for i < n; one layout for all, per-handle n_vec and leading
+ * dimensions (device pointers).  Same arithmetic as h2p_hip_matmul_dev. */
+int h2p_hip_matmul_batch(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
+                         int layout, const int *n_vec,
+                         const double *const *d_X, const int *ldx,
+                         double *const *d_Y, const int *ldy, void *stream);
+
+/* Start CG solves of (A_i + shift_i I) x_i = b_i: r = b - (A + shift I) x0
+ * with x0 = the current contents of d_x[i], stopping once ||r|| <= tol_i ||b||
+ * or after max_iter_i iterations (pcg.c:29-62).  d_b and d_x must stay valid
+ * until the solve is polled done; x is updated in place. */
+int h2p_hip_cg_start(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
+                     const double *const *d_b, double *const *d_x,
+                     const double *shift, const double *tol,
+                     const int *max_iter, void *stream);
+
+/* Advance every listed solve by n_iter iterations of pcg.c:63-123 (no
+ * preconditioner); solves already done are left unchanged. */
+int h2p_hip_cg_step(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
+                    int n_iter, void *stream);
+
+/* Wait for `stream` and report each solve's state: done flag, iterations,
+ * relative residual ||r|| / ||b|| of the recurrence (pcg.c:125-127). */
+int h2p_hip_cg_poll(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
+                    int *done, int *iter, double *relres, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@ checker; the product path never reaches it.
 from __future__ import annotations
 
 import ctypes as ct
+import math
 import os
 import subprocess
 
@@ -186,3 +187,35 @@ def direct_nbody(kid: int, param, coord: np.ndarray, x: np.ndarray, rows: np.nda
                            coord.shape[1], coord.shape[1], x.ctypes.data,
                            dst.ctypes.data, rows.size, rows.size, y.ctypes.data, n_thread)
     return y
+
+
+def pcg(matvec, b: np.ndarray, tol: float, max_iter: int, x0: np.ndarray | None = None):
+    """Unpreconditioned CG of examples/PCG/pcg.c:11-152 (invMx = NULL, so
+    z = r): r = b - A x; stop when ||r|| <= tol ||b|| or after max_iter
+    iterations.  ``matvec`` applies the operator (for the shifted system of
+    examples/SPDHSS-H2/pcg_tests.c:15-21 it adds shift * x itself).
+    Returns (x, flag, relres, iter) with flag 0 on convergence."""
+    n = b.size
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    r = b - matvec(x)                                    # pcg.c:29-37
+    b_2norm = math.sqrt(float(b @ b))                    # pcg.c:43-50
+    r_2norm = math.sqrt(float(r @ r))
+    rn_stop = b_2norm * tol
+    it = 0
+    rho = 1.0
+    p = np.zeros(n)
+    while it < max_iter and r_2norm > rn_stop:          # pcg.c:63
+        z = r                                            # pcg.c:67-68
+        rho0 = rho                                       # pcg.c:76-81
+        rho = float(r @ z)
+        beta = rho / rho0
+        p = z.copy() if it == 0 else z + beta * p        # pcg.c:83-90
+        s = matvec(p)                                    # pcg.c:95
+        alpha = rho / float(p @ s)                       # pcg.c:101-105
+        x = x + alpha * p                                # pcg.c:110-117
+        r = r - alpha * s
+        r_2norm = math.sqrt(float(r @ r))
+        it += 1
+    flag = 0 if r_2norm <= rn_stop else 1                # pcg.c:125-127
+    relres = r_2norm / b_2norm if b_2norm > 0 else float("nan")   # C's 0 / 0
+    return x, flag, relres, it

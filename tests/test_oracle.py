@@ -151,3 +151,34 @@ def test_oracle_aot_equals_jit_and_matmul_equals_matvec(name):
     assert rel(np.stack([oj.matvec(X[:, i], n_thread=4) for i in range(6)], 1), cols) < 1e-14
     assert rel(oa.matmul(X, "col", n_thread=4), cols) < 1e-14
     assert rel(oj.matmul(X, "row", n_thread=4), cols) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# PCG restatement (examples/PCG/pcg.c:11-152), pinned by what CG must do in
+# exact arithmetic and by a direct solve.
+def test_oracle_pcg_finite_termination_on_three_eigenvalues():
+    """An SPD operator with 3 distinct eigenvalues: CG ends in 3 iterations."""
+    rng = np.random.default_rng(1)
+    d = np.repeat([1.0, 4.0, 9.0], 20)
+    Q, _ = np.linalg.qr(rng.standard_normal((60, 60)))
+    A = (Q * d) @ Q.T
+    b = rng.standard_normal(60)
+    x, flag, relres, it = O.pcg(lambda v: A @ v, b, 1e-10, 50)
+    assert flag == 0 and it == 3
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_oracle_pcg_matches_direct_solve_and_edge_cases():
+    rng = np.random.default_rng(2)
+    M = rng.standard_normal((80, 80))
+    A = M @ M.T / 80 + 0.5 * np.eye(80)
+    b = rng.standard_normal(80)
+    x, flag, relres, it = O.pcg(lambda v: A @ v, b, 1e-12, 500)
+    assert flag == 0 and relres <= 1e-12
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(np.linalg.solve(A, b)) < 1e-10
+    # max_iter = 0: no iteration, x0 returned, flag 1 (pcg.c:63, :125)
+    x0, flag0, rr0, it0 = O.pcg(lambda v: A @ v, b, 1e-12, 0)
+    assert it0 == 0 and flag0 == 1 and np.all(x0 == 0) and abs(rr0 - 1.0) < 1e-15
+    # b = 0 with x0 = 0: r = 0 <= tol * 0 stops at once, flag 0
+    xz, flagz, rrz, itz = O.pcg(lambda v: A @ v, np.zeros(80), 1e-6, 10)
+    assert itz == 0 and flagz == 0 and math.isnan(rrz)
