@@ -15,7 +15,8 @@ EXPORTS = ("h2p_hip_upload", "h2p_hip_matvec", "h2p_hip_matvec_dev", "h2p_hip_ma
            "h2p_hip_free", "h2p_hip_last_error", "h2p_hip_direct_nbody",
            "h2p_hip_footprint", "h2p_hip_batch_create", "h2p_hip_batch_free",
            "h2p_hip_matvec_batch", "h2p_hip_matmul_batch", "h2p_hip_cg_start",
-           "h2p_hip_cg_step", "h2p_hip_cg_poll")
+           "h2p_hip_cg_step", "h2p_hip_cg_poll", "h2p_hip_autotune", "h2p_hip_mm_variant")
+MM_VARIANTS = 4
 
 _vp = ct.c_void_p
 
@@ -101,4 +102,8 @@ def bind(lib: ct.CDLL) -> ct.CDLL:
     lib.h2p_hip_cg_step.restype = ct.c_int
     lib.h2p_hip_cg_poll.argtypes = [_vp, ct.c_int, _vp, _vp, _vp, _vp, _vp]
     lib.h2p_hip_cg_poll.restype = ct.c_int
+    lib.h2p_hip_autotune.argtypes = [_vp, ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
+    lib.h2p_hip_autotune.restype = ct.c_int
+    lib.h2p_hip_mm_variant.argtypes = [_vp, ct.c_int, ct.c_int, ct.c_int]
+    lib.h2p_hip_mm_variant.restype = ct.c_int
     return lib

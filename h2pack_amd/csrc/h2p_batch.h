@@ -137,13 +137,13 @@ struct MMCtx {
     int row_major, v0;
 };
 
-template <int KID, int KD>
+template <int KID, int KD, class G>
 __global__ void __launch_bounds__(256) k_mm_b(const BatchItem *__restrict__ items, int n,
                                               const MMCtx *__restrict__ reqs) {
     const int it = find_item(items, n, blockIdx.x);
     const MMCtx &c = reqs[items[it].req];
     const MMTask t = ((const MMTask *)items[it].tasks)[blockIdx.x - items[it].begin];
-    mm_body<KID, KD>(t, c.srcs, c.U, c.UT, c.BD, c.coord, c.CT, c.kp, c.P);
+    mm_body<KID, KD, G>(t, c.srcs, c.U, c.UT, c.BD, c.coord, c.CT, c.kp, c.P);
 }
 
 __global__ void __launch_bounds__(256) k_mm_gather_b(const BatchItem *__restrict__ items, int n,

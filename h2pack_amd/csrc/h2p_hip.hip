@@ -14,6 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -97,9 +100,12 @@ struct h2p_hip_handle {
     MMTask *d_mmt = nullptr;
     MMSrc *d_mms = nullptr;
     std::vector<Level> mm_fwd_levels, mm_bwd_levels;
+    std::vector<MMTask> h_mmt;    // host copies, for the autotuner's histograms
+    std::vector<MMSrc> h_mms;
     Level mm_intmd{0, 0}, mm_leaf{0, 0};
     double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, panel_nvp wide
     int panel_nvp = 0;
+    int mm_var[4][3] = {};        // tile variant per matmul phase and panel width class
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
@@ -245,20 +251,58 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
     return 0;
 }
 
-template <int KID, int KD>
-static void launch_mm(h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
-    if (L.n <= 0) return;
-    hipLaunchKernelGGL((k_mm<KID, KD>), dim3(L.n), dim3(256), 0, s, h->d_mmt + L.off, h->d_mms,
+// matmul launches: kernel dimension and id, panel width class (NV = 16, 32
+// or 64 columns) and tile variant (chosen by h2p_hip_autotune)
+static int nv_class(int nvp) { return nvp <= 16 ? 0 : nvp <= 32 ? 1 : 2; }
+
+template <int KID, int KD, int NV, int V>
+static void launch_mm_v(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
+    hipLaunchKernelGGL((k_mm<KID, KD, MMVar<KD, NV, V>>), dim3(L.n), dim3(256), 0, s, h->d_mmt + L.off, h->d_mms,
                        h->d_U, h->d_UT, (const double *)h->d_blob, h->d_coord, h->CT, h->kp, P);
 }
 
-static void dispatch_mm(h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
-    switch (h->kid) {
-        case KC: launch_mm<KC, 1>(h, L, P, s); break;
-        case KG: launch_mm<KG, 1>(h, L, P, s); break;
-        case KM: launch_mm<KM, 1>(h, L, P, s); break;
-        default: launch_mm<KR, 3>(h, L, P, s); break;
+template <int KID, int KD, int NV>
+static void launch_mm_nv(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s, int v) {
+    switch (v) {
+        case 1: launch_mm_v<KID, KD, NV, 1>(h, L, P, s); break;
+        case 2: launch_mm_v<KID, KD, NV, 2>(h, L, P, s); break;
+        case 3: launch_mm_v<KID, KD, NV, 3>(h, L, P, s); break;
+        default: launch_mm_v<KID, KD, NV, 0>(h, L, P, s); break;
     }
+}
+
+template <int KID, int KD>
+static void launch_mm(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s, int v) {
+    switch (nv_class(P.nvp)) {
+        case 0: launch_mm_nv<KID, KD, 16>(h, L, P, s, v); break;
+        case 1: launch_mm_nv<KID, KD, 32>(h, L, P, s, v); break;
+        default: launch_mm_nv<KID, KD, 64>(h, L, P, s, v); break;
+    }
+}
+
+static void dispatch_mm(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s, int v) {
+    if (L.n <= 0) return;
+    switch (h->kid) {
+        case KC: launch_mm<KC, 1>(h, L, P, s, v); break;
+        case KG: launch_mm<KG, 1>(h, L, P, s, v); break;
+        case KM: launch_mm<KM, 1>(h, L, P, s, v); break;
+        default: launch_mm<KR, 3>(h, L, P, s, v); break;
+    }
+}
+
+// the four phase classes of a matmul, as level lists
+enum { MMP_FWD = 0, MMP_INTMD = 1, MMP_BWD = 2, MMP_LEAF = 3 };
+static std::vector<Level> mm_phase_levels(const h2p_hip_handle *h, int ph) {
+    switch (ph) {
+        case MMP_FWD: return h->mm_fwd_levels;
+        case MMP_INTMD: return {h->mm_intmd};
+        case MMP_BWD: return h->mm_bwd_levels;
+        default: return {h->mm_leaf};
+    }
+}
+
+static void launch_mm_phase(const h2p_hip_handle *h, int ph, const MMPanels &P, hipStream_t s, int v) {
+    for (const Level &L : mm_phase_levels(h, ph)) dispatch_mm(h, L, P, s, v);
 }
 
 extern "C" {
@@ -755,6 +799,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if ((rc = dev_upload(&h->d_contrib, contrib, acct))) return bail(rc);
     if ((rc = dev_upload(&h->d_mmt, mmt, acct))) return bail(rc);
     if ((rc = dev_upload(&h->d_mms, mms, acct))) return bail(rc);
+    h->h_mmt = std::move(mmt);
+    h->h_mms = std::move(mms);
     h->mm_ok = !(mode == H2P_HIP_AOT && prec == H2P_HIP_FP32MIX);
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
@@ -948,16 +994,42 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
         const long long ne = (long long)N * nvp;
         hipLaunchKernelGGL(k_mm_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, d_X, (long long)ldx,
                            rm ? 1 : 0, v0, nv, h->d_fwd, N, P.p[MM_PX], nvp);
-        for (const Level &L : h->mm_fwd_levels) dispatch_mm(h, L, P, s);
-        dispatch_mm(h, h->mm_intmd, P, s);
-        for (const Level &L : h->mm_bwd_levels) dispatch_mm(h, L, P, s);
-        dispatch_mm(h, h->mm_leaf, P, s);
+        const int c = nv_class(nvp);
+        for (int ph = 0; ph < 4; ph++) launch_mm_phase(h, ph, P, s, h->mm_var[ph][c]);
         const long long no = (long long)N * nv;
         hipLaunchKernelGGL(k_mm_scatter, dim3((unsigned)((no + 255) / 256)), dim3(256), 0, s, P.p[MM_PY], nvp,
                            h->d_fwd, N, d_Y, (long long)ldy, rm ? 1 : 0, v0, nv);
         HIP_TRY(hipGetLastError());
     }
     return 0;
+}
+
+// Tile-variant choices shared by handles whose matmul phase has the same
+// kernel, mode, panel width class and block-size histogram.
+static std::mutex g_tune_mu;
+static std::map<std::string, int> g_tune_cache;
+
+static std::string tune_signature(const h2p_hip_handle *h, int ph, int cls, const std::vector<MMTask> &tasks,
+                                  const std::vector<MMSrc> &srcs) {
+    // histogram of (log2 M, log2 K) over the phase's (task, source) products
+    std::map<std::pair<int, int>, long long> hist;
+    for (const Level &L : mm_phase_levels(h, ph))
+        for (int i = L.off; i < L.off + L.n; i++) {
+            const MMTask &t = tasks[i];
+            for (int k = t.sbeg; k < t.send; k++) {
+                const int bm = t.M > 0 ? (int)std::floor(std::log2((double)t.M)) : -1;
+                const int bk = srcs[k].K > 0 ? (int)std::floor(std::log2((double)srcs[k].K)) : -1;
+                hist[{bm, bk}]++;
+            }
+        }
+    std::string sig = std::to_string(h->kid) + "/" + std::to_string(h->mode) + "/" + std::to_string(ph) + "/" +
+                      std::to_string(cls) + ":";
+    for (const auto &e : hist) {
+        // counts on a log2 scale, so nearby histograms share a choice
+        const int c = (int)std::floor(std::log2((double)e.second));
+        sig += std::to_string(e.first.first) + "," + std::to_string(e.first.second) + "," + std::to_string(c) + ";";
+    }
+    return sig;
 }
 
 int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, int ldx, double *Y,
@@ -987,6 +1059,75 @@ int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, in
     (void)hipFree(dX);
     (void)hipFree(dY);
     return rc;
+}
+
+int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double *t_tuned_ms) {
+    if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
+    if (n_vec <= 0) return fail(H2P_HIP_EINVAL, "n_vec must be positive");
+    if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
+    const int nvp = panel_width(n_vec), cls = nv_class(nvp);
+    int rc = ensure_panels(h, nvp);
+    if (rc) return rc;
+    const MMPanels P = panels_of(h, nvp);
+    hipStream_t s = h->own_stream;
+    HIP_TRY(hipMemsetAsync(h->d_panels, 0, sizeof(double) * panel_rows(h->N, h->S) * nvp, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double tdef = 0.0, ttun = 0.0;
+    int err = 0;
+    for (int ph = 0; ph < 4 && !err; ph++) {
+        bool empty = true;
+        for (const Level &L : mm_phase_levels(h, ph)) empty = empty && L.n <= 0;
+        if (empty) continue;
+        const std::string sig = tune_signature(h, ph, cls, h->h_mmt, h->h_mms);
+        double t[H2P_MM_NVARIANT];
+        int best = 0;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_cache.find(sig);
+            if (it != g_tune_cache.end()) {
+                h->mm_var[ph][cls] = it->second;
+                continue;
+            }
+        }
+        const int reps = 3;
+        for (int v = 0; v < H2P_MM_NVARIANT && !err; v++) {
+            launch_mm_phase(h, ph, P, s, v);   // warm-up
+            if (hipEventRecord(e0, s) != hipSuccess) { err = 1; break; }
+            for (int r = 0; r < reps; r++) launch_mm_phase(h, ph, P, s, v);
+            float ms = 0.f;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                err = 1;
+            t[v] = ms / reps;
+            if (t[v] < t[best]) best = v;
+        }
+        if (err) break;
+        h->mm_var[ph][cls] = best;
+        tdef += t[0];
+        ttun += t[best];
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[sig] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err) return fail(H2P_HIP_EDEVICE, "timing the matmul variants failed");
+    HIP_TRY(hipGetLastError());
+    if (t_default_ms) *t_default_ms = tdef;
+    if (t_tuned_ms) *t_tuned_ms = ttun;
+    return 0;
+}
+
+int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant) {
+    if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
+    if (phase < -1 || phase > 3 || n_vec <= 0 || variant < -1 || variant >= H2P_MM_NVARIANT)
+        return fail(H2P_HIP_EINVAL, "bad phase, width or variant");
+    const int cls = nv_class(panel_width(n_vec));
+    if (variant < 0) return h->mm_var[phase < 0 ? 0 : phase][cls];
+    for (int ph = 0; ph < 4; ph++)
+        if (phase < 0 || ph == phase) h->mm_var[ph][cls] = variant;
+    return 0;
 }
 
 int h2p_hip_set_profiling(h2p_hip_handle *h, int on) {
@@ -1499,14 +1640,25 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
         if (lg.grid > 0)
             hipLaunchKernelGGL(k_mm_gather_b, dim3(lg.grid), dim3(256), 0, s, (const BatchItem *)(dbase + lg.items_off),
                                lg.n_items, dctx);
+        int max_nvp = 16;
+        for (const MMCtx &c : ctx) max_nvp = std::max(max_nvp, c.P.nvp);
+        const int ncls = nv_class(max_nvp);
         for (const MMLaunch &ph : phases) {
             const BatchItem *it = (const BatchItem *)(dbase + ph.b.items_off);
             const dim3 grid(ph.b.grid), blk(256);
-            switch (ph.kid) {
-                case KC: hipLaunchKernelGGL((k_mm_b<KC, 1>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
-                case KG: hipLaunchKernelGGL((k_mm_b<KG, 1>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
-                case KM: hipLaunchKernelGGL((k_mm_b<KM, 1>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
-                default: hipLaunchKernelGGL((k_mm_b<KR, 3>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+            switch (ph.kid * 3 + ncls) {
+                case KC * 3 + 0: hipLaunchKernelGGL((k_mm_b<KC, 1, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KC * 3 + 1: hipLaunchKernelGGL((k_mm_b<KC, 1, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KC * 3 + 2: hipLaunchKernelGGL((k_mm_b<KC, 1, MMVar<1, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KG * 3 + 0: hipLaunchKernelGGL((k_mm_b<KG, 1, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KG * 3 + 1: hipLaunchKernelGGL((k_mm_b<KG, 1, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KG * 3 + 2: hipLaunchKernelGGL((k_mm_b<KG, 1, MMVar<1, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KM * 3 + 0: hipLaunchKernelGGL((k_mm_b<KM, 1, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KM * 3 + 1: hipLaunchKernelGGL((k_mm_b<KM, 1, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KM * 3 + 2: hipLaunchKernelGGL((k_mm_b<KM, 1, MMVar<1, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KR * 3 + 0: hipLaunchKernelGGL((k_mm_b<KR, 3, MMVar<3, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                case KR * 3 + 1: hipLaunchKernelGGL((k_mm_b<KR, 3, MMVar<3, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                default: hipLaunchKernelGGL((k_mm_b<KR, 3, MMVar<3, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
             }
         }
         if (ls.grid > 0)

@@ -47,33 +47,54 @@ struct MMPanels {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// Tile geometry: KD = 1: 64 rows x 16 inner; KD = 3: 48 rows (16 points) x 48 inner.
-template <int KD> struct MMGeom;
-template <> struct MMGeom<1> { static constexpr int RC = 64, KC = 16; };
-template <> struct MMGeom<3> { static constexpr int RC = 48, KC = 48; };
+// Tile geometry of one variant: RC rows of the destination per chunk (a
+// multiple of 16, and of 48 for the 3x3 kernel so tiles hold whole points),
+// KC inner entries per LDS stage, NV the widest panel the variant takes.
+// The (RC/16) x (nvp/16) output tiles of a chunk are dealt to the four waves
+// round-robin, so every wave issues MFMAs whatever the panel width.
+template <int RC_, int KC_, int NV_>
+struct MMCfg {
+    static constexpr int RC = RC_, KC = KC_, NV = NV_;
+    static_assert(RC % 16 == 0 && KC % 4 == 0 && NV % 16 == 0 && NV <= 64, "bad tile geometry");
+};
 
 #define H2P_MM_MAXNVP 64
+#define H2P_MM_NVARIANT 4
 
-template <int KID, int KD>
+// The variants the autotuner chooses from, per kernel dimension and panel
+// width class (NV = 16, 32, 64); variant 0 is the default.
+template <int KD, int NV, int V> struct MMVar;
+template <int NV> struct MMVar<1, NV, 0> : MMCfg<64, 32, NV> {};
+template <int NV> struct MMVar<1, NV, 1> : MMCfg<64, 64, NV> {};
+template <int NV> struct MMVar<1, NV, 2> : MMCfg<128, 32, NV> {};
+template <int NV> struct MMVar<1, NV, 3> : MMCfg<32, 64, NV> {};
+template <int NV> struct MMVar<3, NV, 0> : MMCfg<48, 48, NV> {};
+template <int NV> struct MMVar<3, NV, 1> : MMCfg<96, 24, NV> {};
+template <int NV> struct MMVar<3, NV, 2> : MMCfg<96, 48, NV> {};
+template <int NV> struct MMVar<3, NV, 3> : MMCfg<48, 24, NV> {};
+
+template <int KID, int KD, class G>
 __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict__ srcs,
                                         const double *__restrict__ Ublob,
                                         const double *__restrict__ UTblob,
                                         const double *__restrict__ BDblob,
                                         const double *__restrict__ cx, long long cstride,
                                         const KParam &kp, const MMPanels &P) {
-    constexpr int RC = MMGeom<KD>::RC, KC = MMGeom<KD>::KC;
-    constexpr int MT = RC / 16;                 // M tiles per chunk (4 or 3)
+    constexpr int RC = G::RC, KC = G::KC, NV = G::NV;
+    constexpr int MT = RC / 16;                         // M tiles per chunk
+    constexpr int SLOTS = (MT * (NV / 16) + 3) / 4;     // output tiles per wave, at most
     __shared__ double As[RC][KC + 1];
-    __shared__ double Xs[KC][H2P_MM_MAXNVP + 1];
-    const int nvp = P.nvp, NT = nvp >> 4;
+    __shared__ double Xs[KC][NV + 1];
+    const int nvp = P.nvp, NT = nvp >> 4;               // nvp <= NV (host-checked)
+    const int T = MT * NT;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
     double *dst = P.p[t.dst_space] + (long long)t.dst_off * nvp;
     const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
     for (int mc = 0; mc < t.M; mc += RC) {
-        d4 acc[H2P_MM_MAXNVP / 16];
+        d4 acc[SLOTS];
 #pragma unroll
-        for (int q = 0; q < H2P_MM_MAXNVP / 16; q++) acc[q] = (d4){0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < SLOTS; j++) acc[j] = (d4){0.0, 0.0, 0.0, 0.0};
         for (int si = t.sbeg; si < t.send; si++) {
             const MMSrc s = srcs[si];
             const double *X = P.p[s.x_space] + (long long)s.x_off * nvp;
@@ -124,30 +145,32 @@ __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict
                     Xs[k][c] = (kc + k < s.K) ? X[(long long)(kc + k) * nvp + c] : 0.0;
                 }
                 __syncthreads();
-                if (w < MT) {
 #pragma unroll
-                    for (int kk = 0; kk < KC; kk += 4) {
-                        const double a = As[w * 16 + li][kk + lq];
+                for (int j = 0; j < SLOTS; j++) {
+                    const int tt = w + 4 * j;
+                    if (tt < T) {
+                        const int mt = tt / NT, nt = tt - mt * NT;
 #pragma unroll
-                        for (int q = 0; q < H2P_MM_MAXNVP / 16; q++)
-                            if (q < NT)
-                                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Xs[kk + lq][q * 16 + li], acc[q], 0, 0, 0);
+                        for (int kk = 0; kk < KC; kk += 4)
+                            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[mt * 16 + li][kk + lq],
+                                                                          Xs[kk + lq][nt * 16 + li], acc[j], 0, 0, 0);
                     }
                 }
             }
         }
-        // ---- write the chunk: register j of lane (li, lq) holds row 4 j + lq of
-        // the M tile, column li (the f64 16x16x4 output layout)
-        if (w < MT) {
+        // ---- write the chunk: register r of lane (li, lq) holds row 4 r + lq
+        // of its M tile, column li (the f64 16x16x4 output layout)
 #pragma unroll
-            for (int q = 0; q < H2P_MM_MAXNVP / 16; q++) {
-                if (q >= NT) break;
+        for (int j = 0; j < SLOTS; j++) {
+            const int tt = w + 4 * j;
+            if (tt < T) {
+                const int mt = tt / NT, nt = tt - mt * NT;
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int m = mc + w * 16 + 4 * j + lq;
+                for (int r = 0; r < 4; r++) {
+                    const int m = mc + mt * 16 + 4 * r + lq;
                     if (m < t.M) {
-                        double *o = dst + (long long)m * nvp + q * 16 + li;
-                        *o = t.accumulate ? *o + acc[q][j] : acc[q][j];
+                        double *o = dst + (long long)m * nvp + nt * 16 + li;
+                        *o = t.accumulate ? *o + acc[j][r] : acc[j][r];
                     }
                 }
             }
@@ -155,7 +178,7 @@ __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict
     }
 }
 
-template <int KID, int KD>
+template <int KID, int KD, class G>
 __global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
                                             const MMSrc *__restrict__ srcs,
                                             const double *__restrict__ Ublob,
@@ -163,7 +186,7 @@ __global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
                                             const double *__restrict__ BDblob,
                                             const double *__restrict__ cx, long long cstride,
                                             KParam kp, MMPanels P) {
-    mm_body<KID, KD>(tasks[blockIdx.x], srcs, Ublob, UTblob, BDblob, cx, cstride, kp, P);
+    mm_body<KID, KD, G>(tasks[blockIdx.x], srcs, Ublob, UTblob, BDblob, cx, cstride, kp, P);
 }
 
 // Caller panel (original order, row- or column-major, ld) -> PX (permuted,

@@ -153,6 +153,21 @@ class H2MatHIP:
         _check(self.lib.h2p_hip_matmul_dev(self.handle, layout, n_vec, ct.c_void_p(x_ptr), ldx,
                                            ct.c_void_p(y_ptr), ldy, ct.c_void_p(stream)))
 
+    def autotune(self, n_vec: int) -> dict:
+        """Time the matmul tile variants on this matrix for ``n_vec`` columns
+        and keep the fastest per phase (h2p_hip_autotune)."""
+        t0, t1 = ct.c_double(0), ct.c_double(0)
+        _check(self.lib.h2p_hip_autotune(self.handle, int(n_vec), ct.byref(t0), ct.byref(t1)))
+        return {"default_ms": t0.value, "tuned_ms": t1.value,
+                "variants": [self.mm_variant(ph, n_vec) for ph in range(4)]}
+
+    def mm_variant(self, phase: int, n_vec: int, variant: int = -1) -> int:
+        """Pin (variant >= 0) or read (variant -1) a matmul phase's tile variant."""
+        rc = self.lib.h2p_hip_mm_variant(self.handle, int(phase), int(n_vec), int(variant))
+        if rc < 0:
+            _check(rc)
+        return rc
+
     def set_profiling(self, on: bool):
         _check(self.lib.h2p_hip_set_profiling(self.handle, 1 if on else 0))
 

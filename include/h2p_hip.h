@@ -141,6 +141,25 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
                        const double *d_X, int ldx, double *d_Y, int ldy,
                        void *stream);
 
+/* Autotuner of the matmul tiles.  Each matmul phase (upward sweep, coupling,
+ * downward sweep, leaf outputs) runs as grouped MFMA launches whose tile
+ * geometry (rows per chunk, inner entries per LDS stage) is one of
+ * H2P_HIP_MM_VARIANTS compiled variants.  This call times every variant on
+ * the handle's own blocks for panels of n_vec columns and keeps the fastest
+ * per phase; choices are shared, within the process, by handles whose phase
+ * has the same kernel, mode and block-size histogram.  Every variant
+ * computes bitwise the same result.  Optional outputs: summed phase times
+ * of the default variants and of the chosen ones, in milliseconds.
+ * (H2Pack has no counterpart: its matmul tiles are fixed BLAS calls.) */
+#define H2P_HIP_MM_VARIANTS 4
+int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms,
+                     double *t_tuned_ms);
+
+/* Pin the tile variant (0 .. H2P_HIP_MM_VARIANTS-1) of one matmul phase
+ * (0..3, or -1 for all) for panels of n_vec columns; variant -1 returns the
+ * current choice of the phase (phase -1: of phase 0) instead. */
+int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant);
+
 /* Record per-phase HIP events on each call (up to 512 calls between reads;
  * recording does not synchronise). */
 int h2p_hip_set_profiling(h2p_hip_handle *h, int on);
