@@ -419,6 +419,24 @@ def _pad_ext(c, xpt_dim, ext_val):
     return np.concatenate([c, e], axis=0)
 
 
+def _sketch(kid, param, Y, X, rows, vals, nnz):
+    """(S^T K(Y, X))^T for a scalar kernel, S^T row c = (rows, vals)[c*nnz:(c+1)*nnz]
+    (``h2p_host_sketch`` in csrc/h2p_host.c)."""
+    from ._lib import host_lib
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    nx = X.shape[1]
+    out = np.empty((nx, nx))
+    has_param = param is not None and len(param) > 0
+    p = np.ascontiguousarray(param if has_param else [0.0], dtype=np.float64)
+    host_lib().h2p_host_sketch(kid, p.ctypes.data if has_param else None, Y.ctypes.data, Y.shape[1],
+                               X.ctypes.data, nx, nx, r.ctypes.data, v.ctypes.data, int(nnz),
+                               out.ctypes.data, _nthreads())
+    return out
+
+
 def _gen_pp_nlayer(kid, param, pt_dim, xpt_dim, kd, reltol, L1, L2, L3, rng,
                    X0_size=2000, Y0_lsize=4000, max_layer=8, ext_val=0.0):
     """src/H2Pack_gen_proxy_point.c:56-356 with alg = 2 (layer by layer)."""
@@ -431,15 +449,19 @@ def _gen_pp_nlayer(kid, param, pt_dim, xpt_dim, kd, reltol, L1, L2, L3, rng,
     # (1)-(2) kernel matrix and sparse random sketch with 32 +-1 per column.
     # The kernels are symmetric, so K(Y0, X0) is A^T already in row-major
     # order; the sketch is A1 = A S = (S^T A^T)^T.
-    At = kernel_eval(kid, param, Y0, X0)
-    k, n = At.shape
+    k, n = Y0.shape[1] * kd, X0.shape[1] * kd
     nnz_col = min(32, k)
     rows = np.concatenate([rng.choice(k, nnz_col, replace=False) for _ in range(n)])
     cols = np.repeat(np.arange(n), nnz_col)
     vals = rng.integers(0, 2, size=rows.size) * 2.0 - 1.0
-    St = sps.csr_matrix((vals, (cols, rows)), shape=(n, k))
-    A1 = np.asarray(St @ At).T
-    del At
+    if kd == 1:
+        # fused: the sketch straight from kernel values, K(Y0, X0) never formed
+        A1 = _sketch(kid, param, Y0, X0, rows, vals, nnz_col)
+    else:
+        At = kernel_eval(kid, param, Y0, X0)
+        St = sps.csr_matrix((vals, (cols, rows)), shape=(n, k))
+        A1 = np.asarray(St @ At).T
+        del At
     # (3) skeleton of X
     _, skel = id_compress(A1, QR_REL_NRM, reltol * 1e-2, kd, need_U=False)
     Xp = X0[:, skel]

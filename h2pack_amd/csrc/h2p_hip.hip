@@ -124,6 +124,10 @@ struct h2p_hip_handle {
     int use_graph = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // the interior of a matmul chunk (all product phases on the panels), one
+    // graph per panel width; dropped when the panels move or a variant changes
+    struct MMGraph { int nvp; hipGraph_t g; hipGraphExec_t e; };
+    std::vector<MMGraph> mm_graphs;
     // device-side CG state of the batched solver (allocated by the first start)
     double *d_cg = nullptr;       // r | p | s | partials | scalars
     double *cg_r = nullptr, *cg_p = nullptr, *cg_s = nullptr, *cg_part = nullptr;
@@ -143,6 +147,10 @@ static void free_handle(h2p_hip_handle *h) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
+    for (auto &m : h->mm_graphs) {
+        (void)hipGraphExecDestroy(m.e);
+        (void)hipGraphDestroy(m.g);
+    }
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
@@ -303,6 +311,42 @@ static std::vector<Level> mm_phase_levels(const h2p_hip_handle *h, int ph) {
 
 static void launch_mm_phase(const h2p_hip_handle *h, int ph, const MMPanels &P, hipStream_t s, int v) {
     for (const Level &L : mm_phase_levels(h, ph)) dispatch_mm(h, L, P, s, v);
+}
+
+static void drop_mm_graphs(h2p_hip_handle *h) {
+    if (h->mm_graphs.empty()) return;
+    (void)hipDeviceSynchronize();   // a replay may still be running
+    for (auto &m : h->mm_graphs) {
+        (void)hipGraphExecDestroy(m.e);
+        (void)hipGraphDestroy(m.g);
+    }
+    h->mm_graphs.clear();
+}
+
+// all product phases of one matmul chunk with panels P, captured once per
+// panel width and replayed
+static int mm_graph(h2p_hip_handle *h, const MMPanels &P, hipGraphExec_t *out) {
+    for (auto &m : h->mm_graphs)
+        if (m.nvp == P.nvp) {
+            *out = m.e;
+            return 0;
+        }
+    hipStream_t cs = h->own_stream;
+    const int c = nv_class(P.nvp);
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int ph = 0; ph < 4; ph++) launch_mm_phase(h, ph, P, cs, h->mm_var[ph][c]);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    if (e != hipSuccess) return fail(H2P_HIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(H2P_HIP_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    h->mm_graphs.push_back({P.nvp, g, ge});
+    *out = ge;
+    return 0;
 }
 
 extern "C" {
@@ -891,6 +935,7 @@ static int panel_width(int n_vec) { return std::min(H2P_MM_MAXNVP, (n_vec + 15) 
 static int ensure_panels(h2p_hip_handle *h, int nvp) {
     if (h->panel_nvp >= nvp) return 0;
     const size_t rows = panel_rows(h->N, h->S);
+    drop_mm_graphs(h);
     if (h->d_panels) {
         // earlier products on any stream may still read the old panels
         HIP_TRY(hipDeviceSynchronize());
@@ -995,7 +1040,13 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
         hipLaunchKernelGGL(k_mm_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, d_X, (long long)ldx,
                            rm ? 1 : 0, v0, nv, h->d_fwd, N, P.p[MM_PX], nvp);
         const int c = nv_class(nvp);
-        for (int ph = 0; ph < 4; ph++) launch_mm_phase(h, ph, P, s, h->mm_var[ph][c]);
+        if (h->use_graph) {
+            hipGraphExec_t ge = nullptr;
+            if ((rc = mm_graph(h, P, &ge))) return rc;
+            HIP_TRY(hipGraphLaunch(ge, s));
+        } else {
+            for (int ph = 0; ph < 4; ph++) launch_mm_phase(h, ph, P, s, h->mm_var[ph][c]);
+        }
         const long long no = (long long)N * nv;
         hipLaunchKernelGGL(k_mm_scatter, dim3((unsigned)((no + 255) / 256)), dim3(256), 0, s, P.p[MM_PY], nvp,
                            h->d_fwd, N, d_Y, (long long)ldy, rm ? 1 : 0, v0, nv);
@@ -1087,6 +1138,7 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune_cache.find(sig);
             if (it != g_tune_cache.end()) {
+                if (h->mm_var[ph][cls] != it->second) drop_mm_graphs(h);
                 h->mm_var[ph][cls] = it->second;
                 continue;
             }
@@ -1104,6 +1156,7 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
             if (t[v] < t[best]) best = v;
         }
         if (err) break;
+        if (h->mm_var[ph][cls] != best) drop_mm_graphs(h);
         h->mm_var[ph][cls] = best;
         tdef += t[0];
         ttun += t[best];
@@ -1126,7 +1179,10 @@ int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant) {
     const int cls = nv_class(panel_width(n_vec));
     if (variant < 0) return h->mm_var[phase < 0 ? 0 : phase][cls];
     for (int ph = 0; ph < 4; ph++)
-        if (phase < 0 || ph == phase) h->mm_var[ph][cls] = variant;
+        if ((phase < 0 || ph == phase) && h->mm_var[ph][cls] != variant) {
+            drop_mm_graphs(h);
+            h->mm_var[ph][cls] = variant;
+        }
     return 0;
 }
 

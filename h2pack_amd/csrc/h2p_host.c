@@ -118,3 +118,32 @@ void h2p_host_kernel_eval(int kid, const double *param,
         }
     }
 }
+
+/* Sketched kernel block for the numerical proxy points (kd = 1): with
+ * A^T = K(Y, X) (ny x nx) and a sparse sketch S^T whose row c holds nnz
+ * entries (columns rows[c*nnz + t], values vals[c*nnz + t]), returns
+ * out = (S^T A^T)^T, nx x nx row-major:
+ *     out[j][c] = sum_t vals[c*nnz+t] * K(Y[rows[c*nnz+t]], X[j]),
+ * summed in t order.  Computes the sketch without forming K(Y, X). */
+#define SKETCH_CHUNK 2048
+void h2p_host_sketch(int kid, const double *param,
+                     const double *cy, int ldy, const double *cx, int ldx, int nx,
+                     const int *rows, const double *vals, int nnz, double *out, int n_thread)
+{
+    #pragma omp parallel for schedule(static) num_threads(n_thread > 0 ? n_thread : 1)
+    for (int c = 0; c < nx; c++) {
+        double col[SKETCH_CHUNK], kr[SKETCH_CHUNK];
+        for (int j0 = 0; j0 < nx; j0 += SKETCH_CHUNK) {
+            const int n1 = (j0 + SKETCH_CHUNK < nx ? j0 + SKETCH_CHUNK : nx) - j0;
+            for (int j = 0; j < n1; j++) col[j] = 0.0;
+            for (int t = 0; t < nnz; t++) {
+                const int r = rows[(long)c * nnz + t];
+                const double v = vals[(long)c * nnz + t];
+                /* K(Y[r], X[j0 .. j0+n1)) as one row (leading dimension 0) */
+                eval_rows(kid, param, cy, ldy, r, r + 1, cx + j0, ldx, n1, kr - 0, 0);
+                for (int j = 0; j < n1; j++) col[j] += v * kr[j];
+            }
+            for (int j = 0; j < n1; j++) out[(long)(j0 + j) * nx + c] = col[j];
+        }
+    }
+}

@@ -172,3 +172,28 @@ def test_graph_replays_match_eager_path(mode):
         m.info()
         m.set_profiling(False)
         assert np.array_equal(y_graph, y_eager), it
+
+
+@pytest.mark.parametrize("name", ["gaussian", "rpy"])
+def test_matmul_graph_replays_match_eager_handle(name, monkeypatch):
+    """Each matmul chunk's product phases are a captured hipGraph (one per
+    panel width), re-captured when a tile variant changes; every replay must
+    equal a handle uploaded with graphs switched off, bit for bit, with
+    unrelated device work between replays."""
+    import torch
+    h2, _ = problem(name)
+    g = H2MatHIP(h2, mode="jit")
+    monkeypatch.setenv("H2P_HIP_NO_GRAPH", "1")
+    e = H2MatHIP(h2, mode="jit")
+    monkeypatch.delenv("H2P_HIP_NO_GRAPH")
+    rng = np.random.default_rng(22)
+    junk = torch.randn(1 << 20, device="cuda:0", dtype=torch.float64)
+    for it, n_vec in enumerate([8, 8, 32, 8, 70]):
+        if it == 3:
+            g.mm_variant(-1, 8, 2)          # new variant: the width-16 graph is re-captured
+        X = rng.standard_normal((h2.krnl_mat_size, n_vec))
+        Yg = g.matmul(X, "col")
+        junk.mul_(1.0001).add_(0.5)
+        torch.cuda.synchronize()
+        Ye = e.matmul(X, "col")
+        assert np.array_equal(Yg, Ye), it
