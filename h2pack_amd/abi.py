@@ -52,7 +52,7 @@ class Info(ct.Structure):
         ("mode", ct.c_int), ("prec", ct.c_int),
         ("t_fwd", ct.c_double), ("t_blocks", ct.c_double), ("t_reduce", ct.c_double),
         ("t_bwd", ct.c_double), ("t_final", ct.c_double), ("t_total", ct.c_double),
-        ("n_profiled", ct.c_int),
+        ("n_profiled", ct.c_int), ("mv_variant", ct.c_int),
     ]
 
     def as_dict(self):

@@ -100,12 +100,14 @@ struct h2p_hip_handle {
     MMTask *d_mmt = nullptr;
     MMSrc *d_mms = nullptr;
     std::vector<Level> mm_fwd_levels, mm_bwd_levels;
-    std::vector<MMTask> h_mmt;    // host copies, for the autotuner's histograms
+    std::vector<BlkTask> h_blk;   // host copies, for the autotuner's histograms
+    std::vector<MMTask> h_mmt;
     std::vector<MMSrc> h_mms;
     Level mm_intmd{0, 0}, mm_leaf{0, 0};
     double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, panel_nvp wide
     int panel_nvp = 0;
     int mm_var[4][3] = {};        // tile variant per matmul phase and panel width class
+    int mv_var = 0;               // variant of the matvec block launch (0 default, 1 occupancy)
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
@@ -173,10 +175,15 @@ static void launch_build(h2p_hip_handle *h, hipStream_t s) {
                            h->d_coord, h->CT, h->kp, (double *)h->d_blob);
 }
 
+// occupancy of the variant-1 kernels: FP64 JIT 4 waves per SIMD, FP32 JIT 5
 template <int KID, typename T>
 static void launch_jit2(h2p_hip_handle *h, hipStream_t s) {
-    hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
-                       h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+    if (h->mv_var == 1)
+        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, sizeof(T) == 8 ? 4 : 5>), dim3(h->n_blk), dim3(256), 0, s,
+                           h->d_blk, h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+    else
+        hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                           h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
 }
 
 static void dispatch_jit(h2p_hip_handle *h, hipStream_t s) {
@@ -198,22 +205,36 @@ static void dispatch_build(h2p_hip_handle *h, hipStream_t s) {
     }
 }
 
+// The grouped B/D launch of a matvec (variant 1: the occupancy kernels, AOT
+// FP64 at 5 waves per SIMD, AOT FP32 at 8).
+static void launch_blocks(h2p_hip_handle *h, hipStream_t s) {
+    const dim3 g(h->n_blk), b(256);
+    if (h->mode == H2P_HIP_AOT && h->prec == H2P_HIP_FP32MIX) {
+        if (h->mv_var == 1)
+            hipLaunchKernelGGL((k_blocks_aot_occ<float, 8>), g, b, 0, s, h->d_blk, (const float *)h->d_blob,
+                               h->d_xin, h->d_arena);
+        else
+            hipLaunchKernelGGL(k_blocks_aot<float>, g, b, 0, s, h->d_blk, (const float *)h->d_blob, h->d_xin,
+                               h->d_arena);
+    } else if (h->mode == H2P_HIP_AOT) {
+        if (h->mv_var == 1)
+            hipLaunchKernelGGL((k_blocks_aot_occ<double, 5>), g, b, 0, s, h->d_blk, (const double *)h->d_blob,
+                               h->d_xin, h->d_arena);
+        else
+            hipLaunchKernelGGL(k_blocks_aot<double>, g, b, 0, s, h->d_blk, (const double *)h->d_blob, h->d_xin,
+                               h->d_arena);
+    } else {
+        dispatch_jit(h, s);
+    }
+}
+
 // Interior of a matvec: upward sweep, grouped B/D launch, skeleton sums,
 // downward sweep (reads xin[0, N) = px, writes yp / y1 / arena).
 static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
     for (const Level &L : h->fwd_levels)
         hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
     if (ev) (void)hipEventRecord(ev[1], s);
-    if (h->n_blk > 0) {
-        if (h->mode == H2P_HIP_AOT && h->prec == H2P_HIP_FP32MIX)
-            hipLaunchKernelGGL(k_blocks_aot<float>, dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
-                               (const float *)h->d_blob, h->d_xin, h->d_arena);
-        else if (h->mode == H2P_HIP_AOT)
-            hipLaunchKernelGGL(k_blocks_aot<double>, dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
-                               (const double *)h->d_blob, h->d_xin, h->d_arena);
-        else
-            dispatch_jit(h, s);
-    }
+    if (h->n_blk > 0) launch_blocks(h, s);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (h->n_red > 0)
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
@@ -234,6 +255,15 @@ static int ensure_graph(h2p_hip_handle *h) {
     h->graph = g;
     HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
     return 0;
+}
+
+static void drop_matvec_graph(h2p_hip_handle *h) {
+    if (!h->graph_exec) return;
+    (void)hipDeviceSynchronize();   // a replay may still be running
+    (void)hipGraphExecDestroy(h->graph_exec);
+    (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
 }
 
 // One matvec on device pointers (x, y in the caller's order).
@@ -845,6 +875,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if ((rc = dev_upload(&h->d_mms, mms, acct))) return bail(rc);
     h->h_mmt = std::move(mmt);
     h->h_mms = std::move(mms);
+    h->h_blk = blk;
     h->mm_ok = !(mode == H2P_HIP_AOT && prec == H2P_HIP_FP32MIX);
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
@@ -1112,9 +1143,67 @@ int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, in
     return rc;
 }
 
+// n_vec == 0: the matvec's grouped block launch (default or occupancy
+// variant; the RPY JIT kernel has one variant).
+static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tuned_ms) {
+    if (t_default_ms) *t_default_ms = 0.0;
+    if (t_tuned_ms) *t_tuned_ms = 0.0;
+    if (h->n_blk == 0 || (h->mode == H2P_HIP_JIT && h->kid == KR)) return 0;
+    std::map<std::pair<int, int>, long long> hist;
+    for (const BlkTask &t : h->h_blk) {
+        const int a = t.nrow > 0 ? (int)std::floor(std::log2((double)t.nrow)) : -1;
+        const int b = t.ncol > 0 ? (int)std::floor(std::log2((double)t.ncol)) : -1;
+        hist[{a, b}]++;
+    }
+    std::string sig = "mv/" + std::to_string(h->kid) + "/" + std::to_string(h->mode) + "/" + std::to_string(h->prec) + ":";
+    for (const auto &e : hist)
+        sig += std::to_string(e.first.first) + "," + std::to_string(e.first.second) + "," +
+               std::to_string((int)std::floor(std::log2((double)e.second))) + ";";
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(sig);
+        if (it != g_tune_cache.end()) {
+            if (h->mv_var != it->second) drop_matvec_graph(h);
+            h->mv_var = it->second;
+            return 0;
+        }
+    }
+    hipStream_t s = h->own_stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int saved = h->mv_var, reps = 3;
+    double t[2] = {0.0, 0.0};
+    int err = 0;
+    for (int v = 0; v < 2 && !err; v++) {
+        h->mv_var = v;
+        launch_blocks(h, s);   // warm-up
+        if (hipEventRecord(e0, s) != hipSuccess) { err = 1; break; }
+        for (int r = 0; r < reps; r++) launch_blocks(h, s);
+        float ms = 0.f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            err = 1;
+        t[v] = ms / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->mv_var = saved;
+    if (err) return fail(H2P_HIP_EDEVICE, "timing the matvec variants failed");
+    const int best = t[1] < t[0] ? 1 : 0;
+    if (best != h->mv_var) drop_matvec_graph(h);
+    h->mv_var = best;
+    if (t_default_ms) *t_default_ms = t[0];
+    if (t_tuned_ms) *t_tuned_ms = t[best];
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[sig] = best;
+    return 0;
+}
+
 int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double *t_tuned_ms) {
     if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
-    if (n_vec <= 0) return fail(H2P_HIP_EINVAL, "n_vec must be positive");
+    if (n_vec < 0) return fail(H2P_HIP_EINVAL, "n_vec must not be negative");
+    if (n_vec == 0) return autotune_matvec(h, t_default_ms, t_tuned_ms);
     if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
     const int nvp = panel_width(n_vec), cls = nv_class(nvp);
     int rc = ensure_panels(h, nvp);
@@ -1174,6 +1263,15 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
 
 int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant) {
     if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
+    if (phase == 4) {   // the matvec's block launch
+        if (variant < -1 || variant > 1) return fail(H2P_HIP_EINVAL, "bad matvec variant");
+        if (variant < 0) return h->mv_var;
+        if (h->mode == H2P_HIP_JIT && h->kid == KR && variant != 0)
+            return fail(H2P_HIP_ENOSYS, "the RPY JIT block launch has one variant");
+        if (variant != h->mv_var) drop_matvec_graph(h);
+        h->mv_var = variant;
+        return 0;
+    }
     if (phase < -1 || phase > 3 || n_vec <= 0 || variant < -1 || variant >= H2P_MM_NVARIANT)
         return fail(H2P_HIP_EINVAL, "bad phase, width or variant");
     const int cls = nv_class(panel_width(n_vec));
@@ -1224,6 +1322,7 @@ int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info) {
         h->info.n_profiled = h->ev_used;
         h->ev_used = 0;
     }
+    h->info.mv_variant = h->mv_var;
     *info = h->info;
     return 0;
 }

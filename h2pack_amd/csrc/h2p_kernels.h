@@ -621,6 +621,23 @@ __device__ __forceinline__ void blocks_jit2_body(const BlkTask &t, const double 
     }
 }
 
+// Occupancy variant of the grouped launch (the autotuner's alternative for
+// the matvec blocks): the same body compiled for at least WPE waves per SIMD,
+// so with fewer registers per lane.  Same arithmetic, same results.
+template <int KID, typename T, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_blocks_jit2_occ(const BlkTask *__restrict__ tasks, const double *__restrict__ cx, long long cstride,
+                  double scale, double diag, const double *__restrict__ xin, double *__restrict__ arena) {
+    blocks_jit2_body<KID, T>(tasks[blockIdx.x], cx, cstride, scale, diag, xin, arena);
+}
+
+template <typename TS, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_blocks_aot_occ(const BlkTask *__restrict__ tasks, const TS *__restrict__ blob,
+                 const double *__restrict__ xin, double *__restrict__ arena) {
+    blocks_aot_body<TS>(tasks[blockIdx.x], blob, xin, arena);
+}
+
 template <int KID, typename T>
 __global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks,
                                                      const double *__restrict__ cx,

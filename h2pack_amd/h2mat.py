@@ -154,12 +154,17 @@ class H2MatHIP:
                                            ct.c_void_p(y_ptr), ldy, ct.c_void_p(stream)))
 
     def autotune(self, n_vec: int) -> dict:
-        """Time the matmul tile variants on this matrix for ``n_vec`` columns
-        and keep the fastest per phase (h2p_hip_autotune)."""
+        """Time the kernel variants on this matrix and keep the fastest
+        (h2p_hip_autotune): the matmul tiles per phase for ``n_vec`` columns,
+        or with ``n_vec`` = 0 the matvec's grouped block launch."""
         t0, t1 = ct.c_double(0), ct.c_double(0)
         _check(self.lib.h2p_hip_autotune(self.handle, int(n_vec), ct.byref(t0), ct.byref(t1)))
-        return {"default_ms": t0.value, "tuned_ms": t1.value,
-                "variants": [self.mm_variant(ph, n_vec) for ph in range(4)]}
+        out = {"default_ms": t0.value, "tuned_ms": t1.value}
+        if n_vec == 0:
+            out["variants"] = [self.info()["mv_variant"]]
+        else:
+            out["variants"] = [self.mm_variant(ph, n_vec) for ph in range(4)]
+        return out
 
     def mm_variant(self, phase: int, n_vec: int, variant: int = -1) -> int:
         """Pin (variant >= 0) or read (variant -1) a matmul phase's tile variant."""

@@ -112,6 +112,7 @@ typedef struct {
      * since the previous h2p_hip_get_info (HIP events on the call's stream) */
     double t_fwd, t_blocks, t_reduce, t_bwd, t_final, t_total;
     int n_profiled;
+    int mv_variant;      /* variant of the matvec block launch (h2p_hip_autotune) */
 } h2p_hip_info;
 
 /* Mirror an H2 representation into HBM.  mode = H2P_HIP_AOT / _JIT,
@@ -150,14 +151,18 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
  * has the same kernel, mode and block-size histogram.  Every variant
  * computes bitwise the same result.  Optional outputs: summed phase times
  * of the default variants and of the chosen ones, in milliseconds.
- * (H2Pack has no counterpart: its matmul tiles are fixed BLAS calls.) */
+ * n_vec = 0 tunes the matvec instead: its grouped B/D launch has a default
+ * and an occupancy variant (the same body compiled for more waves per SIMD,
+ * with fewer registers), again with identical results.
+ * (H2Pack has no counterpart: its products are fixed BLAS / SIMD loops.) */
 #define H2P_HIP_MM_VARIANTS 4
 int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms,
                      double *t_tuned_ms);
 
 /* Pin the tile variant (0 .. H2P_HIP_MM_VARIANTS-1) of one matmul phase
  * (0..3, or -1 for all) for panels of n_vec columns; variant -1 returns the
- * current choice of the phase (phase -1: of phase 0) instead. */
+ * current choice of the phase (phase -1: of phase 0) instead.  Phase 4 is
+ * the matvec's block launch (variants 0 and 1; n_vec is ignored). */
 int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant);
 
 /* Record per-phase HIP events on each call (up to 512 calls between reads;

@@ -45,3 +45,23 @@ def test_tuning_is_shared_by_identical_histograms():
     ta = a.autotune(16)
     tb = b.autotune(16)    # same histogram: taken from the cache, nothing timed
     assert tb["variants"] == ta["variants"] and tb["default_ms"] == 0.0
+
+
+@pytest.mark.parametrize("name,mode,prec", [("gaussian", "jit", "fp64"), ("gaussian", "jit", "fp32mix"),
+                                            ("coulomb", "aot", "fp64"), ("coulomb", "aot", "fp32mix")])
+def test_matvec_variants_give_the_same_result(name, mode, prec):
+    """The matvec's block launch has an occupancy variant; both variants and
+    the tuned choice give bitwise the same y (graph replays included)."""
+    h2, _ = problem(name)
+    m = H2MatHIP(h2, mode=mode, prec=prec, host_blocks=False)
+    x = np.random.default_rng(5).random(h2.krnl_mat_size) - 0.5
+    y0 = m.matvec(x)
+    m.mm_variant(4, 1, 1)                     # the occupancy variant, pinned
+    assert m.mm_variant(4, 1) == 1
+    assert np.array_equal(m.matvec(x), y0)
+    m.mm_variant(4, 1, 0)
+    t = m.autotune(0)
+    print(f"[{name} {mode} {prec}] matvec tune {t}", flush=True)
+    assert t["variants"][0] in (0, 1)
+    assert np.array_equal(m.matvec(x), y0)
+    assert np.array_equal(m.matvec(x), y0)
