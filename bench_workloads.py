@@ -189,8 +189,17 @@ def run(args):
         res = {}
         for mode in ("jit", "aot"):
             m = H2MatHIP(h2, mode=mode, host_blocks=False)
-            dt = _time(lambda: m.matmul_dev(abi.COL_MAJOR, nv, Xd.data_ptr(), Nk, Yd.data_ptr(), Nk, sptr), K, W, sync)
+            mm = lambda: m.matmul_dev(abi.COL_MAJOR, nv, Xd.data_ptr(), Nk, Yd.data_ptr(), Nk, sptr)   # noqa: E731
+            dt_default = _time(mm, K, W, sync)
+            Y0 = Yd.cpu().numpy().T
+            t_t = time.perf_counter()
+            tune = m.autotune(nv)
+            tune["tune_s"] = time.perf_counter() - t_t
+            dt = _time(mm, K, W, sync)
             Y = Yd.cpu().numpy().T
+            log(f"[bench] rpy {mode}: tile variants {tune['variants']} (phases {tune['default_ms']:.1f} -> "
+                f"{tune['tuned_ms']:.1f} ms), matmul {dt_default * 1e3:.1f} -> {dt * 1e3:.1f} ms, "
+                f"identical: {bool(np.array_equal(Y0, Y))}")
             xv = torch.empty(Nk, dtype=torch.float64, device=dev); yv = torch.empty_like(xv)
             def cols():
                 for v in range(nv):
@@ -204,6 +213,8 @@ def run(args):
             info = m.info()
             flops = 2.0 * nv * (2 * info["U_elems"] + 2 * 9 * info["B_pairs"] + 2 * 9 * info["D_pairs"])
             res[mode] = {"matmuls_per_s": 1.0 / dt, "ms_per_matmul": dt * 1e3,
+                         "ms_per_matmul_default_tiles": dt_default * 1e3, "autotune": tune,
+                         "tuned_equals_default_bitwise": bool(np.array_equal(Y0, Y)),
                          "ms_per_32_matvecs": dtv * 1e3, "speedup_vs_matvec_loop": dtv / dt,
                          "model_TFLOPs": flops / dt / 1e12, "rel_err_vs_matvec_cols": max(errs)}
             log(f"[bench] rpy {mode}: {dt * 1e3:.2f} ms per 32-column matmul ({flops / dt / 1e12:.2f} TFLOP/s model), "
