@@ -236,6 +236,20 @@ def main():
                          f"{el:.1f}s, oracle/h2oracle.c (OpenMP, no BLAS)"}
         log(f"[bench] oracle CPU: {rate:.3f} matvec/s on {nt} threads; GPU vs oracle rel err {err_oracle:.3e}")
 
+    # HBM bytes per launch of the named kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (null when absent)
+    def traffic(kernel):
+        try:
+            with open(os.path.join(ROOT, "profiles", "gaussian_1M_traffic.json")) as f:
+                t = json.load(f)
+            if N != 1_000_000:
+                return None
+            return t["kernels"][kernel]["hbm_bytes"]
+        except (OSError, KeyError, ValueError):
+            return None
+
+    if aot and "roofline" in aot:
+        aot["roofline"]["traffic"] = traffic(aot["roofline"]["kernel"])
     flops = info_j["jit_flops_model"]
     ach = flops / (info_j["t_blocks"] * 1e-3) / 1e12
     out = {
@@ -255,10 +269,11 @@ def main():
                    "N": N, "mode": "jit", "precision": "fp64", "kernel": "gaussian", "reltol": 1e-6,
                    "leaf": 300, "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": ach / FP64_PEAK_TFLOPS,
                      "kernel": ("k_blocks_jit2_occ<1, double, 4>" if tune_j["variants"][0] == 1
                                 else "k_blocks_jit2<1, double>"), "kernel_ms": info_j["t_blocks"],
                      "flops_per_launch": flops, "block_variant": tune_j["variants"][0],
+                     "traffic_source": "profiles/gaussian_1M_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "note": "achieved = H2Pack JIT_flops model (14 flops per Gaussian pair) / kernel time; "
                              "FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak"},
         "cpu_baseline": cpu,
@@ -272,6 +287,7 @@ def main():
                "adm_pairs": h2.n_r_adm_pair, "inadm_pairs": h2.n_r_inadm_pair,
                "B_pairs": info_j["B_pairs"], "D_pairs": info_j["D_pairs"]},
     }
+    out["roofline"]["traffic"] = traffic(out["roofline"]["kernel"])
     print(json.dumps(out), flush=True)
 
 

@@ -69,6 +69,11 @@ def run_serving(args):
     kw = {"max_iter": args.cg_max_iter}
     workers = max(1, args.build_workers)
     threads = max(1, 16 // workers)
+    # the workers inherit these at start: bounded BLAS / OpenMP threads that
+    # sleep instead of spinning when idle, so the processes do not starve
+    # each other
+    os.environ.update({"OMP_WAIT_POLICY": "PASSIVE", "OPENBLAS_NUM_THREADS": str(threads),
+                       "OMP_NUM_THREADS": str(threads), "H2P_BUILD_THREADS": str(threads)})
     t0 = time.perf_counter()
     reqs = [None] * len(specs)
     with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
@@ -78,6 +83,8 @@ def run_serving(args):
             reqs[futs[f]] = r
             log(f"[bench] request {r.rid}: {r.kind} N={r.N} built in {tb:.1f}s")
     t_build = time.perf_counter() - t0
+    for k in ("OMP_WAIT_POLICY", "OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "H2P_BUILD_THREADS"):
+        os.environ.pop(k, None)
     import torch
     free, total = torch.cuda.mem_get_info()
     budget = int(0.8 * free)
