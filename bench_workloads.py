@@ -73,7 +73,7 @@ def run_serving(args):
     # sleep instead of spinning when idle, so the processes do not starve
     # each other
     os.environ.update({"OMP_WAIT_POLICY": "PASSIVE", "OPENBLAS_NUM_THREADS": str(threads),
-                       "OMP_NUM_THREADS": str(threads), "H2P_BUILD_THREADS": str(threads)})
+                       "H2P_BUILD_THREADS": str(threads)})
     t0 = time.perf_counter()
     reqs = [None] * len(specs)
     with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
@@ -83,7 +83,7 @@ def run_serving(args):
             reqs[futs[f]] = r
             log(f"[bench] request {r.rid}: {r.kind} N={r.N} built in {tb:.1f}s")
     t_build = time.perf_counter() - t0
-    for k in ("OMP_WAIT_POLICY", "OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "H2P_BUILD_THREADS"):
+    for k in ("OMP_WAIT_POLICY", "OPENBLAS_NUM_THREADS", "H2P_BUILD_THREADS"):
         os.environ.pop(k, None)
     import torch
     free, total = torch.cuda.mem_get_info()
