@@ -102,7 +102,8 @@ def run_serving(args):
     for name, prec, slots in (("batched_fp32mix", "fp32mix", args.slots), ("alone_fp64", "fp64", 1)):
         rs = fresh()
         srv = Server(hbm_budget=budget, max_active=slots, prec=prec, cg_chunk=8)
-        st = srv.run(rs, realtime=True)
+        log(f"[bench] serving {len(rs)} requests: {name} ({prec}, up to {slots} at once)")
+        st = srv.run(rs, realtime=True, log=log)
         srv.close()
         cg = [r for r in rs if r.kind == "cg"]
         d = st.as_dict()

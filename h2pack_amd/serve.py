@@ -132,9 +132,11 @@ class Server:
             b = torch.zeros_like(a)
         return _Active(req, mat, a, b)
 
-    def run(self, requests: list[Request], realtime: bool = True) -> ServeStats:
+    def run(self, requests: list[Request], realtime: bool = True, log=None,
+            log_every: float = 15.0) -> ServeStats:
         """Serve ``requests`` (arrival times in seconds from the start; with
-        ``realtime`` False every request is available at once)."""
+        ``realtime`` False every request is available at once).  ``log``, if
+        given, receives a progress line every ``log_every`` seconds."""
         torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in pending:
@@ -148,8 +150,14 @@ class Server:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0   # noqa: E731
         i = 0
+        t_log = 0.0
         while i < len(pending) or queue or cg or mm_inflight:
             t = now()
+            if log and t - t_log >= log_every:
+                t_log = t
+                log(f"[serve] t={t:.1f}s done {st.n_done}/{len(pending)} active cg {len(cg)} "
+                    f"products in flight {sum(len(g) for g, _ in mm_inflight)} queued {len(queue)} "
+                    f"bytes {used / 1e9:.1f} GB")
             while i < len(pending) and (not realtime or pending[i].arrival <= t):
                 queue.append(pending[i])
                 i += 1
