@@ -36,6 +36,7 @@ import scipy.linalg as sla
 import scipy.sparse as sps
 
 ALPHA_H2 = 0.999999          # src/H2Pack_config.h:103
+ALPHA_HSS = -0.000001        # src/H2Pack_config.h:104
 QR_RANK, QR_REL_NRM, QR_ABS_NRM = 0, 1, 2   # src/H2Pack_config.h:97-101
 
 KRNL_COULOMB, KRNL_GAUSSIAN, KRNL_MATERN32, KRNL_RPY = 0, 1, 2, 3
@@ -174,6 +175,7 @@ class H2Pack:
         self.QR_stop_tol = float(QR_stop_param)
         self.max_child = 1 << pt_dim
         self.is_RPY = 0
+        self.is_HSS = 0
         self.BD_JIT = 1
         self.root_enbox = None
         self.B_data = None
@@ -320,12 +322,22 @@ def partition_points(h2: H2Pack, coord: np.ndarray, max_leaf_points: int = 0,
     _calc_reduced_adm_pairs(h2)
 
 
-def _calc_reduced_adm_pairs(h2: H2Pack) -> None:
+def _box_admissible(b0: np.ndarray, b1: np.ndarray, pt_dim: int, alpha: float) -> np.ndarray:
+    """H2P_check_box_admissible (src/H2Pack_partition.c) on rows of boxes:
+    admissible when along some dimension the centre distance is at least
+    alpha * min(size) + (size0 + size1) / 2."""
+    r0, r1 = b0[..., pt_dim:], b1[..., pt_dim:]
+    c0 = b0[..., :pt_dim] + 0.5 * r0
+    c1 = b1[..., :pt_dim] + 0.5 * r1
+    return np.any(np.abs(c0 - c1) >= alpha * np.minimum(r0, r1) + 0.5 * (r0 + r1), axis=-1)
+
+
+def _calc_reduced_adm_pairs(h2: H2Pack, alpha: float = ALPHA_H2) -> None:
     """Reduced (in)admissible pairs, src/H2Pack_partition.c:327-432.
 
     Same recursion rules, evaluated breadth-first with numpy.  The pair sets
     equal the reference's; their order is breadth-first instead of
-    depth-first.
+    depth-first.  ``alpha`` is ALPHA_H2, or ALPHA_HSS for the HSS pairs.
     """
     pt_dim = h2.pt_dim
     ch = h2.children.reshape(h2.n_node, h2.max_child)
@@ -348,11 +360,7 @@ def _calc_reduced_adm_pairs(h2: H2Pack) -> None:
             new0.append(kids[ii]); new1.append(kids[jj])
         # distinct pairs
         if p0.size:
-            b0, b1 = box[p0], box[p1]
-            r0, r1 = b0[:, pt_dim:], b1[:, pt_dim:]
-            c0 = b0[:, :pt_dim] + 0.5 * r0
-            c1 = b1[:, :pt_dim] + 0.5 * r1
-            adm = np.any(np.abs(c0 - c1) >= ALPHA_H2 * np.minimum(r0, r1) + 0.5 * (r0 + r1), axis=1)
+            adm = _box_admissible(box[p0], box[p1], pt_dim, alpha)
             adm0.append(p0[adm]); adm1.append(p1[adm])
             q0, q1 = p0[~adm], p1[~adm]
             l0 = nchild[q0] == 0
@@ -596,6 +604,95 @@ def _build_UJ(h2: H2Pack, pp) -> None:
             h2.U_ncol[node] = 0
 
 
+# --------------------------------------------------------------------------
+# HSS mode (H2P_run_HSS): weak admissibility and hybrid bases
+# --------------------------------------------------------------------------
+def _calc_hss_pairs(h2: H2Pack) -> None:
+    """H2P_HSS_calc_adm_inadm_pairs (src/H2Pack_partition.c:507-540): reduced
+    pairs with ALPHA_HSS replace the pair lists, whose minimum admissible
+    level must be 1.  The H2 pairs are kept as h2.H2_* for reference."""
+    h2.H2_r_adm_pairs, h2.H2_r_inadm_pairs = h2.r_adm_pairs, h2.r_inadm_pairs
+    h2.H2_min_adm_level = h2.min_adm_level
+    _calc_reduced_adm_pairs(h2, ALPHA_HSS)
+    if h2.min_adm_level != 1:
+        raise ValueError("HSS minimal admissible level should be 1")
+
+
+def _calc_node_inadm_lists(h2: H2Pack) -> list:
+    """H2P_calc_node_inadm_lists (src/H2Pack_partition.c:434-505): for each
+    node at a level >= 1, the nodes of its level and the leaves above it that
+    are not H2-admissible but HSS-admissible (its near neighbours)."""
+    pt_dim, nl = h2.pt_dim, h2.n_leaf_node
+    box = h2.enbox.reshape(h2.n_node, 2 * pt_dim)
+    leaves = h2.leaf_nodes
+    lists = [np.zeros(0, np.int64) for _ in range(h2.n_node)]
+    for lv in range(h2.max_level, h2.min_adm_level - 1, -1):
+        nodes = h2.level_nodes[lv * nl: lv * nl + h2.level_n_node[lv]].astype(np.int64)
+        targets = np.concatenate([nodes, leaves[h2.node_level[leaves] < lv].astype(np.int64)])
+        bn, bt = box[nodes][:, None, :], box[targets][None, :, :]
+        near = (~_box_admissible(bn, bt, pt_dim, ALPHA_H2) & _box_admissible(bn, bt, pt_dim, ALPHA_HSS)
+                & (nodes[:, None] != targets[None, :]))
+        for k, n in enumerate(nodes):
+            lists[n] = targets[near[k]]
+    return lists
+
+
+def _build_UJ_hss(h2: H2Pack, pp, seed: int = 2718) -> None:
+    """H2P_build_HSS_UJ_hybrid (src/H2Pack_build.c:272-600): bottom-up from
+    the leaves, each node compresses K(its candidate points, [candidate points
+    of its near neighbours, its level's proxy points]) (both centred on the
+    node's box), sketched by a Gaussian random matrix when it has more than
+    twice as many columns as rows, by ID.  Candidate points: a leaf's points,
+    or the union of its children's skeletons.  A neighbour contributes its
+    candidate points, fixed before any node of the level is compressed (the
+    reference's parallel loop may read some neighbours already compressed;
+    this order is deterministic)."""
+    kid, param, kd = h2.krnl_id, h2.krnl_param, h2.krnl_dim
+    nn, pt_dim, nl = h2.n_node, h2.pt_dim, h2.n_leaf_node
+    ch = h2.children.reshape(nn, h2.max_child)
+    box = h2.enbox.reshape(nn, 2 * pt_dim)
+    pc = h2.pt_cluster.reshape(-1, 2)
+    lists = _calc_node_inadm_lists(h2)
+    h2.node_inadm_lists = lists
+    h2.node_n_r_inadm = np.asarray([l.size for l in lists], dtype=np.int32)
+    U = [np.zeros((0, 0)) for _ in range(nn)]
+    J = [np.zeros(0, dtype=np.int64) for _ in range(nn)]
+    cand = [None] * nn
+    for n in h2.leaf_nodes:
+        cand[n] = np.arange(pc[n, 0], pc[n, 1] + 1, dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    for lv in range(h2.max_level, h2.min_adm_level - 1, -1):
+        nodes = h2.level_nodes[lv * nl: lv * nl + h2.level_n_node[lv]]
+        for n in nodes:
+            if h2.n_child[n]:
+                cand[n] = np.concatenate([J[c] for c in ch[n, :h2.n_child[n]]])
+        for n in nodes:
+            center = box[n, :pt_dim] + 0.5 * box[n, pt_dim:]
+            pts = h2.coord[:, cand[n]].copy()
+            pts[:pt_dim] -= center[:, None]
+            nb = [cand[k] for k in lists[n]]
+            nb = np.concatenate(nb) if nb else np.zeros(0, dtype=np.int64)
+            far = h2.coord[:, nb].copy()
+            far[:pt_dim] -= center[:, None]
+            ppl = pp[lv] if lv < len(pp) else np.zeros((h2.xpt_dim, 0))
+            far = np.concatenate([far, ppl], axis=1)
+            if far.shape[1] == 0 or cand[n].size == 0:
+                U[n] = np.zeros((cand[n].size * kd, 0))
+                J[n] = np.zeros(0, dtype=np.int64)
+                continue
+            A = kernel_eval(kid, param, pts, far)
+            if A.shape[1] > 2 * A.shape[0]:
+                A = A @ rng.standard_normal((A.shape[1], A.shape[0]))
+            Un, sub = id_compress(A, h2.QR_stop_type, h2.QR_stop_tol, kd)
+            U[n] = np.ascontiguousarray(Un)
+            J[n] = cand[n][sub]
+    h2.U = U
+    h2.J = J
+    h2.J_coord = [np.ascontiguousarray(h2.coord[:, j]) for j in J]
+    h2.U_nrow = np.asarray([u.shape[0] for u in U], dtype=np.int32)
+    h2.U_ncol = np.asarray([u.shape[1] for u in U], dtype=np.int32)
+
+
 def _coo_to_csr(n_node, pi, pj, pv):
     """H2P_int_COO_to_CSR (src/H2Pack_utils.c:265-290): columns sorted per row."""
     order = np.lexsort((pj, pi))
@@ -704,7 +801,10 @@ def build(h2: H2Pack, pp, BD_JIT: int, krnl_id: int, krnl_param,
     h2.BD_JIT = int(BD_JIT)
     h2.krnl_bimv_flops = KRNL_BIMV_FLOPS[h2.krnl_id]
     t0 = time.perf_counter()
-    _build_UJ(h2, pp)
+    if getattr(h2, "is_HSS", 0):
+        _build_UJ_hss(h2, pp)
+    else:
+        _build_UJ(h2, pp)
     t1 = time.perf_counter()
     _gen_B_metadata(h2)
     _gen_D_metadata(h2)
@@ -740,9 +840,14 @@ def uniform_cube_points(n: int, density: float, seed: int, xpt_dim: int = 3,
 
 def build_h2(coord: np.ndarray, krnl_id: int, krnl_param, reltol: float,
              max_leaf_points: int, BD_JIT: int = 1, pp_kind: str = "auto",
-             host_blocks: bool | None = None, max_leaf_size: float = 0.0) -> H2Pack:
+             host_blocks: bool | None = None, max_leaf_size: float = 0.0,
+             hss: bool = False) -> H2Pack:
     """Partition + proxy points + build in one call (the sequence of
-    examples/example_H2.c:45-88)."""
+    examples/example_H2.c:45-88; with ``hss`` the HSS sequence of
+    examples/example_HSS.c:51-80, H2P_run_HSS before the partition).  An HSS
+    representation carries its HSS pair lists and minimum admissible level in
+    the ordinary fields, which is what H2P_matvec reads in HSS mode
+    (src/H2Pack_matvec.c:101, 308, 592, 773, 904, 1009)."""
     import time
     kd = KRNL_DIM[krnl_id]
     h2 = H2Pack(pt_dim=3, krnl_dim=kd, QR_stop_type=QR_REL_NRM, QR_stop_param=reltol)
@@ -762,6 +867,9 @@ def build_h2(coord: np.ndarray, krnl_id: int, krnl_param, reltol: float,
     t2 = time.perf_counter()
     h2.pp = pp
     h2.pp_kind = pp_kind
+    if hss:
+        h2.is_HSS = 1
+        _calc_hss_pairs(h2)
     build(h2, pp, BD_JIT, krnl_id, krnl_param, host_blocks=host_blocks)
     h2.timers["partition"] = t1 - t0
     h2.timers["proxy_points"] = t2 - t1

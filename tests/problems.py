@@ -18,6 +18,10 @@ SPECS = {
     "tiny_coulomb": (C.KRNL_COULOMB, [], 900, 1.0, 40, 1e-8, "surface"),
     "tiny_gaussian": (C.KRNL_GAUSSIAN, [0.5], 900, 4.0, 40, 1e-8, "numerical"),
     "tiny_rpy": (C.KRNL_RPY, [1.0], 500, 0.03, 30, 1e-8, "surface"),
+    # HSS mode (weak admissibility, hybrid bases)
+    "hss_coulomb": (C.KRNL_COULOMB, [], 3000, 1.0, 100, 1e-6, "numerical", {"hss": True}),
+    "hss_gaussian": (C.KRNL_GAUSSIAN, [1.0], 3000, 10.0, 100, 1e-6, "numerical", {"hss": True}),
+    "tiny_hss": (C.KRNL_GAUSSIAN, [0.5], 900, 4.0, 40, 1e-8, "numerical", {"hss": True}),
 }
 
 
@@ -33,9 +37,10 @@ def coords_for(kid, n, density, seed):
 
 @functools.lru_cache(maxsize=None)
 def problem(name: str, seed: int = 1234):
-    kid, param, n, dens, leaf, tol, ppk = SPECS[name]
+    kid, param, n, dens, leaf, tol, ppk, *extra = SPECS[name]
     coord = coords_for(kid, n, dens, seed)
-    h2 = C.build_h2(coord, kid, param, tol, leaf, BD_JIT=0, pp_kind=ppk, host_blocks=True)
+    h2 = C.build_h2(coord, kid, param, tol, leaf, BD_JIT=0, pp_kind=ppk, host_blocks=True,
+                    **(extra[0] if extra else {}))
     return h2, coord
 
 

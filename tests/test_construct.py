@@ -36,7 +36,7 @@ def test_tree_partitions_points(name):
     assert np.array_equal(h2.bwd_pmt_idx[h2.fwd_pmt_idx], np.arange(h2.krnl_mat_size))
 
 
-@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "tiny_rpy"])
+@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "tiny_rpy", "tiny_hss"])
 def test_block_partition_covers_matrix_once(name):
     """Diagonal leaf blocks, inadmissible pairs (both orientations) and the
     point sets of admissible pairs (both orientations) tile the matrix
@@ -56,7 +56,7 @@ def test_block_partition_covers_matrix_once(name):
     assert cover.min() == 1 and cover.max() == 1
 
 
-@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "coulomb", "rpy"])
+@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "coulomb", "rpy", "tiny_hss"])
 def test_U_has_identity_rows_on_skeleton(name):
     h2, _ = problem(name)
     kd = h2.krnl_dim
@@ -116,3 +116,30 @@ def test_host_kernel_matches_closed_forms():
     assert np.allclose(C.kernel_eval(C.KRNL_GAUSSIAN, [0.5], c0, c1), [[np.exp(-2.0), 1.0]])
     s3 = np.sqrt(3.0) * 2.0 * 0.7
     assert np.allclose(C.kernel_eval(C.KRNL_MATERN32, [0.7], c0, c1), [[(1 + s3) * np.exp(-s3), 1.0]])
+
+
+@pytest.mark.parametrize("name", ["tiny_hss", "hss_coulomb"])
+def test_hss_pairs_and_neighbour_lists(name):
+    """HSS mode (H2P_HSS_calc_adm_inadm_pairs, H2P_calc_node_inadm_lists):
+    weak admissibility makes every pair of distinct siblings admissible, so
+    the minimum admissible level is 1 and only leaves keep dense blocks; a
+    node's neighbours are the nodes of its level (and leaves above it) that
+    touch it."""
+    h2, _ = problem(name)
+    assert h2.is_HSS == 1 and h2.min_adm_level == 1
+    assert h2.n_r_inadm_pair == 0
+    ch = h2.children.reshape(h2.n_node, -1)
+    adm = {tuple(sorted(p)) for p in h2.r_adm_pairs.reshape(-1, 2).tolist()}
+    for n in range(h2.n_node):
+        kids = ch[n, :h2.n_child[n]].tolist()
+        for i in range(len(kids)):
+            for j in range(i + 1, len(kids)):
+                assert tuple(sorted((kids[i], kids[j]))) in adm
+    assert len(adm) == sum(k * (k - 1) // 2 for k in h2.n_child.tolist())
+    box = h2.enbox.reshape(h2.n_node, -1)
+    for n, lst in enumerate(h2.node_inadm_lists):
+        for k in lst.tolist():
+            assert k != n and n in h2.node_inadm_lists[k].tolist() or h2.n_child[k] == 0
+            gap = np.abs((box[n, :3] + 0.5 * box[n, 3:]) - (box[k, :3] + 0.5 * box[k, 3:])) \
+                - 0.5 * (box[n, 3:] + box[k, 3:])
+            assert gap.max() < 1e-6 * box[n, 3:].max() + 1e-9     # touching, not separated

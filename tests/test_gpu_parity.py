@@ -197,3 +197,26 @@ def test_matmul_graph_replays_match_eager_handle(name, monkeypatch):
         torch.cuda.synchronize()
         Ye = e.matmul(X, "col")
         assert np.array_equal(Yg, Ye), it
+
+
+@pytest.mark.parametrize("name", ["hss_coulomb", "hss_gaussian"])
+@pytest.mark.parametrize("mode", ["jit", "aot"])
+def test_hss_mode_matches_oracle(name, mode):
+    """HSS representations (weak admissibility, hybrid bases; the HSS pair
+    lists in the ordinary fields, as H2P_matvec reads them in HSS mode) run
+    through the same engine: FP64 matvec and matmul against the oracle to
+    1e-12, FP32-mixed within the build tolerance."""
+    h2, _ = problem(name)
+    x = _x(h2)
+    orc = O.OracleH2(h2, BD_JIT=1)
+    y_ref = orc.matvec(x, n_thread=8)
+    m = H2MatHIP(h2, mode=mode, host_blocks=False)
+    err = rel(m.matvec(x), y_ref)
+    X = np.random.default_rng(9).standard_normal((h2.krnl_mat_size, 5))
+    err_mm = rel(m.matmul(X, "col"), orc.matmul(X, "col", n_thread=8))
+    m32 = H2MatHIP(h2, mode=mode, prec="fp32mix", host_blocks=False)
+    err32 = rel(m32.matvec(x), y_ref)
+    print(f"[{name} {mode}] HSS rel err vs oracle: matvec {err:.3e}, matmul {err_mm:.3e}, "
+          f"fp32mix {err32:.3e}", flush=True)
+    assert err < FP64_TOL and err_mm < FP64_TOL
+    assert err32 < 1e-6

@@ -23,7 +23,7 @@ from h2pack_amd import construct as C
 from tests.problems import dense_h2, problem, rel
 
 
-@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "tiny_rpy"])
+@pytest.mark.parametrize("name", ["tiny_coulomb", "tiny_gaussian", "tiny_rpy", "tiny_hss"])
 @pytest.mark.parametrize("jit", [0, 1])
 def test_oracle_matvec_equals_dense_assembly(name, jit):
     h2, _ = problem(name)
@@ -182,3 +182,16 @@ def test_oracle_pcg_matches_direct_solve_and_edge_cases():
     # b = 0 with x0 = 0: r = 0 <= tol * 0 stops at once, flag 0
     xz, flagz, rrz, itz = O.pcg(lambda v: A @ v, np.zeros(80), 1e-6, 10)
     assert itz == 0 and flagz == 0 and math.isnan(rrz)
+
+
+@pytest.mark.parametrize("name", ["hss_coulomb", "hss_gaussian"])
+def test_hss_representation_accuracy(name):
+    """HSS mode against the dense kernel matrix: weak admissibility keeps the
+    error within a few hundred times the ID tolerance (reltol 1e-6)."""
+    h2, coord = problem(name)
+    x = np.random.default_rng(6).random(h2.krnl_mat_size) - 0.5
+    y = O.OracleH2(h2, BD_JIT=1).matvec(x, n_thread=4)
+    K = C.kernel_eval(h2.krnl_id, h2.krnl_param, coord, coord)
+    err = rel(y, K @ x)
+    print(f"[{name}] HSS rel err vs dense = {err:.2e}")
+    assert err < 5e-4
