@@ -100,3 +100,30 @@ def test_upload_from_file_gives_the_same_device_product(mode, tmp_path):
     a = H2MatHIP(h2, mode=mode).matvec(x)
     b = H2MatHIP(g, mode=mode).matvec(x)
     assert np.array_equal(a, b)
+
+
+def test_layout_matches_the_reference_converter(tmp_path):
+    """Fixtures from tests/golden/make_file_io_vectors.py: the JSON files are
+    the output of the reference's own converter (examples/meta_txt_to_json.py)
+    for a 120-point Coulomb representation.  Loading them and storing again
+    must reproduce both JSON files value for value and the binary bit for
+    bit; the loaded representation must be the kernel matrix to its build
+    tolerance."""
+    gold = os.path.join(os.path.dirname(__file__), "golden", "file_io")
+    blob = np.load(os.path.join(gold, "coulomb120_blocks.npy"))
+    binf = str(tmp_path / "gold.bin")
+    blob.tofile(binf)
+    meta_f, aux_f = os.path.join(gold, "coulomb120_meta.json"), os.path.join(gold, "coulomb120_aux.json")
+    g = h2file.load(meta_f, aux_f, binf, C.KRNL_COULOMB, [], BD_JIT=0)
+    paths = _paths(tmp_path)
+    h2file.store(g, *paths)
+    assert json.load(open(paths[0])) == json.load(open(meta_f))
+    assert json.load(open(paths[1])) == json.load(open(aux_f))
+    assert np.array_equal(np.fromfile(paths[2], dtype=np.float64), blob)
+    # the representation itself: within its build tolerance of the dense matrix
+    aux = json.load(open(aux_f))
+    coord0 = np.asarray(aux["point_coordinate"]).reshape(-1, 3).T
+    x = np.random.default_rng(5).random(g.krnl_mat_size) - 0.5
+    y = O.OracleH2(g, BD_JIT=0).matvec(x, n_thread=1)
+    K = C.kernel_eval(C.KRNL_COULOMB, [], coord0, coord0)
+    assert np.linalg.norm(y - K @ x) / np.linalg.norm(K @ x) < 1e-3
