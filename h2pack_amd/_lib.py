@@ -29,6 +29,10 @@ def host_lib() -> ct.CDLL:
         lib.h2p_host_sketch.argtypes = [
             ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int, ct.c_int,
             ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int]
+        lib.h2p_host_rpy_ewald.restype = ct.c_int
+        lib.h2p_host_rpy_ewald.argtypes = [
+            ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
+            ct.c_int, ct.c_int, ct.c_void_p, ct.c_long, ct.c_int]
         _host = lib
     return _host
 

@@ -40,6 +40,7 @@ class H2PackDesc(ct.Structure):
         ("fwd_pmt_idx", _vp), ("bwd_pmt_idx", _vp),
         ("B_nrow", _vp), ("B_ncol", _vp), ("B_ptr", _vp), ("B_data", _vp),
         ("D_nrow", _vp), ("D_ncol", _vp), ("D_ptr", _vp), ("D_data", _vp),
+        ("periodic", ct.c_int), ("per_adm_shifts", _vp), ("per_inadm_shifts", _vp), ("per_blk", _vp),
     ]
 
 

@@ -37,6 +37,10 @@ import scipy.sparse as sps
 
 ALPHA_H2 = 0.999999          # src/H2Pack_config.h:103
 ALPHA_HSS = -0.000001        # src/H2Pack_config.h:104
+# the 27 lattice offsets of the periodic mode, in the order of
+# per_lattices_3d (src/H2Pack_typedef.c:151-179)
+PER_LATTICES_3D = np.array([[i, j, k] for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)],
+                           dtype=np.float64)
 QR_RANK, QR_REL_NRM, QR_ABS_NRM = 0, 1, 2   # src/H2Pack_config.h:97-101
 
 KRNL_COULOMB, KRNL_GAUSSIAN, KRNL_MATERN32, KRNL_RPY = 0, 1, 2, 3
@@ -176,6 +180,7 @@ class H2Pack:
         self.max_child = 1 << pt_dim
         self.is_RPY = 0
         self.is_HSS = 0
+        self.is_RPY_Ewald = 0
         self.BD_JIT = 1
         self.root_enbox = None
         self.B_data = None
@@ -183,6 +188,13 @@ class H2Pack:
         self.krnl_id = None
         self.krnl_param = None
         self.timers = {}
+
+    def run_RPY_Ewald(self):
+        # H2P_run_RPY_Ewald, src/H2Pack_typedef.c:182-208: periodic RPY with
+        # the radius as extended coordinate and the 27 near lattices
+        self.is_RPY_Ewald = 1
+        self.xpt_dim = self.pt_dim + 1
+        self.per_lattices = PER_LATTICES_3D.copy()
 
     def run_RPY(self):
         # H2P_run_RPY, src/H2Pack_typedef.c:130-137: extended coordinate with radius
@@ -319,7 +331,10 @@ def partition_points(h2: H2Pack, coord: np.ndarray, max_leaf_points: int = 0,
     mc[:, 1] = kd * (pc[:, 1] + 1) - 1
     h2.mat_cluster = mc.reshape(-1).astype(np.int32)
     h2.krnl_mat_size = kd * n_point
-    _calc_reduced_adm_pairs(h2)
+    if getattr(h2, "is_RPY_Ewald", 0):
+        _calc_reduced_adm_pairs_per(h2)
+    else:
+        _calc_reduced_adm_pairs(h2)
 
 
 def _box_admissible(b0: np.ndarray, b1: np.ndarray, pt_dim: int, alpha: float) -> np.ndarray:
@@ -403,6 +418,89 @@ def _calc_reduced_adm_pairs(h2: H2Pack, alpha: float = ALPHA_H2) -> None:
         h2.min_adm_level = h2.max_level
     h2.r_adm_pairs = h2.r_adm_pairs.astype(np.int32)
     h2.r_inadm_pairs = h2.r_inadm_pairs.astype(np.int32)
+
+
+def _calc_reduced_adm_pairs_per(h2: H2Pack) -> None:
+    """Reduced pairs of the periodic mode, H2P_calc_reduced_adm_pairs_per and
+    the lattice loop of H2P_partition_points_periodic
+    (src/H2Pack_partition_periodic.c:25-145, 300-340): for every lattice
+    offset l, the root is paired with its image shifted by l times the root
+    box width, and the pair recursion runs with the second node shifted.
+    Pairs are one-sided (node0 receives); an unshifted distinct pair is
+    recorded in both orientations.  Depth-first, in the reference's order.
+    min_adm_level is 0."""
+    pt_dim = h2.pt_dim
+    ch = h2.children.reshape(h2.n_node, h2.max_child)
+    nchild = h2.n_child
+    box = h2.enbox.reshape(h2.n_node, 2 * pt_dim)
+    adm, inadm = [], []         # (n0, n1, lattice)
+
+    def rec(n0, n1, shift, lid):
+        has_shift = bool(np.any(np.abs(shift) > 1e-10))
+        if n0 == n1 and not has_shift:
+            if nchild[n0] == 0:
+                return
+            kids = ch[n0, :nchild[n0]]
+            for c in kids:
+                rec(c, c, shift, lid)
+            for i in range(kids.size):
+                for j in range(i + 1, kids.size):
+                    rec(kids[i], kids[j], shift, lid)
+            return
+        b1 = box[n1].copy()
+        b1[:pt_dim] += shift
+        if _box_admissible(box[n0], b1, pt_dim, ALPHA_H2):
+            adm.append((n0, n1, lid))
+            if not has_shift:
+                adm.append((n1, n0, lid))
+            return
+        l0, l1 = nchild[n0] == 0, nchild[n1] == 0
+        if l0 and l1:
+            inadm.append((n0, n1, lid))
+            if not has_shift:
+                inadm.append((n1, n0, lid))
+            return
+        if l0:
+            for c in ch[n1, :nchild[n1]]:
+                rec(n0, c, shift, lid)
+            return
+        if l1:
+            for c in ch[n0, :nchild[n0]]:
+                rec(c, n1, shift, lid)
+            return
+        for a in ch[n0, :nchild[n0]]:
+            for b in ch[n1, :nchild[n1]]:
+                rec(a, b, shift, lid)
+
+    width = box[h2.root_idx, pt_dim:]
+    shifts = h2.per_lattices * width[None, :]
+    for lid in range(shifts.shape[0]):
+        rec(h2.root_idx, h2.root_idx, shifts[lid], lid)
+    A = np.asarray(adm, dtype=np.int64).reshape(-1, 3)
+    I = np.asarray(inadm, dtype=np.int64).reshape(-1, 3)
+    h2.r_adm_pairs = A[:, :2].reshape(-1).astype(np.int32)
+    h2.r_inadm_pairs = I[:, :2].reshape(-1).astype(np.int32)
+    h2.per_adm_shifts = np.ascontiguousarray(shifts[A[:, 2]]) if A.size else np.zeros((0, pt_dim))
+    h2.per_inadm_shifts = np.ascontiguousarray(shifts[I[:, 2]]) if I.size else np.zeros((0, pt_dim))
+    h2.n_r_adm_pair = int(A.shape[0])
+    h2.n_r_inadm_pair = int(I.shape[0])
+    h2.min_adm_level = 0
+
+
+def rpy_ewald_eval(c0: np.ndarray, c1: np.ndarray, L: float, xi: float, nr: int, nk: int) -> np.ndarray:
+    """Periodic RPY kernel block by Ewald summation (``h2p_host_rpy_ewald``,
+    restating RPY_Ewald_eval_std, src/H2Pack_3D_kernels.h:961-1210); c0, c1
+    are (4, n) with the radius in row 3.  No 1/(6 pi eta) factor."""
+    from ._lib import host_lib
+    c0 = np.ascontiguousarray(c0, dtype=np.float64)
+    c1 = np.ascontiguousarray(c1, dtype=np.float64)
+    n0, n1 = c0.shape[1], c1.shape[1]
+    out = np.empty((3 * n0, 3 * n1))
+    rc = host_lib().h2p_host_rpy_ewald(c0.ctypes.data, n0, n0, c1.ctypes.data, n1, n1, float(L), float(xi),
+                                       int(nr), int(nk), out.ctypes.data, 3 * n1, _nthreads())
+    if rc != 0:
+        raise ValueError("bad Ewald parameters (nk must not exceed nr)")
+    return out
 
 
 # --------------------------------------------------------------------------
@@ -718,11 +816,15 @@ def _gen_B_metadata(h2: H2Pack) -> None:
     h2.B_ncol = (b * kd).astype(np.int32)
     sizes = h2.B_nrow.astype(np.int64) * h2.B_ncol.astype(np.int64)
     h2.B_ptr = np.concatenate(([0], np.cumsum(sizes))).astype(np.uint64)
+    directed = bool(getattr(h2, "is_RPY_Ewald", 0))   # periodic: node0 receives only
     h2.node_n_r_adm = (np.bincount(n0, minlength=h2.n_node)
-                       + np.bincount(n1, minlength=h2.n_node)).astype(np.int32)
+                       + (0 if directed else np.bincount(n1, minlength=h2.n_node))).astype(np.int32)
     m = n0.size
-    pi = np.concatenate([n0, n1]); pj = np.concatenate([n1, n0])
-    pv = np.concatenate([np.arange(1, m + 1), -np.arange(1, m + 1)])
+    if directed:
+        pi, pj, pv = n0, n1, np.arange(1, m + 1)
+    else:
+        pi = np.concatenate([n0, n1]); pj = np.concatenate([n1, n0])
+        pv = np.concatenate([np.arange(1, m + 1), -np.arange(1, m + 1)])
     h2.B_p2i_rowptr, h2.B_p2i_colidx, h2.B_p2i_val = _coo_to_csr(h2.n_node, pi, pj, pv)
     # statistics (src/H2Pack_build.c:755-758)
     h2.B_size = int(sizes.sum())
@@ -743,9 +845,14 @@ def _gen_D_metadata(h2: H2Pack) -> None:
     sizes = r * c
     h2.D_ptr = np.concatenate(([0], np.cumsum(sizes))).astype(np.uint64)
     nl, m = leaves.size, pairs.shape[0]
-    pi = np.concatenate([leaves, pairs[:, 0], pairs[:, 1]])
-    pj = np.concatenate([leaves, pairs[:, 1], pairs[:, 0]])
-    pv = np.concatenate([np.arange(1, nl + 1), nl + np.arange(1, m + 1), -(nl + np.arange(1, m + 1))])
+    if getattr(h2, "is_RPY_Ewald", 0):   # periodic: one-sided pairs
+        pi = np.concatenate([leaves, pairs[:, 0]])
+        pj = np.concatenate([leaves, pairs[:, 1]])
+        pv = np.concatenate([np.arange(1, nl + 1), nl + np.arange(1, m + 1)])
+    else:
+        pi = np.concatenate([leaves, pairs[:, 0], pairs[:, 1]])
+        pj = np.concatenate([leaves, pairs[:, 1], pairs[:, 0]])
+        pv = np.concatenate([np.arange(1, nl + 1), nl + np.arange(1, m + 1), -(nl + np.arange(1, m + 1))])
     h2.D_p2i_rowptr, h2.D_p2i_colidx, h2.D_p2i_val = _coo_to_csr(h2.n_node, pi, pj, pv)
     h2.D_size = int(sizes.sum())
     h2.JIT_D_pairs = float((npt[leaves] ** 2).sum() + (npt[pairs[:, 0]] * npt[pairs[:, 1]]).sum())
@@ -873,4 +980,39 @@ def build_h2(coord: np.ndarray, krnl_id: int, krnl_param, reltol: float,
     build(h2, pp, BD_JIT, krnl_id, krnl_param, host_blocks=host_blocks)
     h2.timers["partition"] = t1 - t0
     h2.timers["proxy_points"] = t2 - t1
+    return h2
+
+
+def build_periodic(coord: np.ndarray, unit_cell, eta: float, ewald, reltol: float,
+                   max_leaf_points: int, max_leaf_size: float = 0.0) -> H2Pack:
+    """Periodic RPY representation (H2P_run_RPY_Ewald, H2P_partition_points_
+    periodic, surface proxy points, H2P_build_periodic; JIT only, as in the
+    reference).  ``coord`` is (4, n) with the radius in row 3; ``unit_cell``
+    = (corner x, y, z, side x, y, z), the root box; ``eta`` the RPY kernel
+    parameter; ``ewald`` = (L, xi, nr, nk) of the Ewald kernel.  The root's
+    periodic block (H2P_build_periodic_block, src/H2Pack_build_periodic.c:
+    15-60) is the Ewald kernel on the root skeleton minus the plain kernel
+    over the 27 near lattices; it lands in h2.per_blk.  As in the reference
+    the Ewald kernel has no 1/(6 pi eta) factor, so the plain and periodic
+    parts agree in scale for eta = 1/(6 pi)."""
+    h2 = H2Pack(pt_dim=3, krnl_dim=3, QR_stop_type=QR_REL_NRM, QR_stop_param=reltol)
+    h2.run_RPY_Ewald()
+    h2.root_enbox = np.asarray(unit_cell, dtype=np.float64)
+    partition_points(h2, coord, max_leaf_points, max_leaf_size)
+    ext = float(np.max(coord[3]))
+    pp = generate_proxy_point_surface(h2, surface_pp_count(reltol), ext_val=ext)
+    h2.pp, h2.pp_kind = pp, "surface"
+    build(h2, pp, 1, KRNL_RPY, [eta], host_blocks=False)
+    # periodic block of the root
+    root = h2.root_idx
+    Jc = h2.J_coord[root]
+    L, xi, nr, nk = ewald
+    per = rpy_ewald_eval(Jc, Jc, L, xi, int(nr), int(nk))
+    width = h2.enbox.reshape(h2.n_node, -1)[root, 3:6]
+    for lat in h2.per_lattices:
+        s = Jc.copy()
+        s[:3] += (lat * width)[:, None]
+        per -= kernel_eval(KRNL_RPY, [eta], Jc, s)
+    h2.per_blk = per
+    h2.ewald_param = (float(L), float(xi), int(nr), int(nk))
     return h2

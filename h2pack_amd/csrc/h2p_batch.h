@@ -29,6 +29,7 @@ struct ReqCtx {
     const void *blob;
     const int *fwd;
     const long long *contrib;
+    const double *shifts;    // shift table (entry 0: zero)
     int N, pad;
     // this call
     const double *x;
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(256) k_blocks_jit_b(const BatchItem *__restric
     const int it = find_item(items, n, blockIdx.x);
     const ReqCtx &c = reqs[items[it].req];
     const BlkTask t = ((const BlkTask *)items[it].tasks)[blockIdx.x - items[it].begin];
-    blocks_jit_body<KID, KD>(t, c.coord, c.CT, c.kp, c.xin, c.arena);
+    blocks_jit_body<KID, KD>(t, c.coord, c.CT, c.kp, c.shifts, c.xin, c.arena);
 }
 
 template <typename TS>
@@ -125,6 +126,7 @@ __global__ void __launch_bounds__(256) k_leaf_final2_b(const BatchItem *__restri
 // request of the batch, each with its own panels.
 struct MMCtx {
     const MMSrc *srcs;
+    const double *shifts;
     const double *U, *UT, *BD, *coord;
     long long CT;
     KParam kp;
@@ -143,7 +145,7 @@ __global__ void __launch_bounds__(256) k_mm_b(const BatchItem *__restrict__ item
     const int it = find_item(items, n, blockIdx.x);
     const MMCtx &c = reqs[items[it].req];
     const MMTask t = ((const MMTask *)items[it].tasks)[blockIdx.x - items[it].begin];
-    mm_body<KID, KD, G>(t, c.srcs, c.U, c.UT, c.BD, c.coord, c.CT, c.kp, c.P);
+    mm_body<KID, KD, G>(t, c.srcs, c.U, c.UT, c.BD, c.coord, c.CT, c.kp, c.shifts, c.P);
 }
 
 __global__ void __launch_bounds__(256) k_mm_gather_b(const BatchItem *__restrict__ items, int n,

@@ -108,6 +108,12 @@ struct h2p_hip_handle {
     int panel_nvp = 0;
     int mm_var[4][3] = {};        // tile variant per matmul phase and panel width class
     int mv_var = 0;               // variant of the matvec block launch (0 default, 1 occupancy)
+    // periodic mode: shift table (x, y, z per entry; entry 0 is zero) and the
+    // root's periodic block (also the matmul's dense source for the root)
+    int periodic = 0;
+    double *d_shift = nullptr;
+    double *d_per = nullptr;
+    int per_r = 0, per_soff = 0;
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
@@ -145,7 +151,7 @@ static void free_handle(h2p_hip_handle *h) {
     void *ptrs[] = {h->d_coord, h->d_U, h->d_UT, h->d_blob, h->d_xin, h->d_y1, h->d_arena,
                     h->d_fwd, h->d_blk, h->d_fwdt, h->d_bwdt, h->d_red, h->d_leaf,
                     h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp, h->d_mmt, h->d_mms, h->d_panels,
-                    h->d_cg};
+                    h->d_cg, h->d_shift, h->d_per};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : h->ev_pool) (void)hipEventDestroy(e);
@@ -162,7 +168,7 @@ static void free_handle(h2p_hip_handle *h) {
 template <int KID, int KD>
 static void launch_jit(h2p_hip_handle *h, hipStream_t s) {
     hipLaunchKernelGGL((k_blocks_jit<KID, KD>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
-                       h->d_coord, h->CT, h->kp, h->d_xin, h->d_arena);
+                       h->d_coord, h->CT, h->kp, h->d_shift, h->d_xin, h->d_arena);
 }
 
 template <int KID, int KD>
@@ -238,6 +244,9 @@ static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
     if (ev) (void)hipEventRecord(ev[2], s);
     if (h->n_red > 0)
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
+    if (h->periodic)
+        hipLaunchKernelGGL(k_root_block, dim3((h->per_r + 3) / 4), dim3(256), 0, s, h->d_per, h->per_r,
+                           h->d_xin + h->N + h->per_soff, h->d_y1 + h->per_soff);
     if (ev) (void)hipEventRecord(ev[3], s);
     for (const Level &L : h->bwd_levels)
         hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
@@ -295,8 +304,10 @@ static int nv_class(int nvp) { return nvp <= 16 ? 0 : nvp <= 32 ? 1 : 2; }
 
 template <int KID, int KD, int NV, int V>
 static void launch_mm_v(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
+    // dense B/D sources: the stored blocks, or in periodic mode the root's block
+    const double *bd = h->periodic ? h->d_per : (const double *)h->d_blob;
     hipLaunchKernelGGL((k_mm<KID, KD, MMVar<KD, NV, V>>), dim3(L.n), dim3(256), 0, s, h->d_mmt + L.off, h->d_mms,
-                       h->d_U, h->d_UT, (const double *)h->d_blob, h->d_coord, h->CT, h->kp, P);
+                       h->d_U, h->d_UT, bd, h->d_coord, h->CT, h->kp, h->d_shift, P);
 }
 
 template <int KID, int KD, int NV>
@@ -405,6 +416,12 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if (g->n_point <= 0 || g->krnl_mat_size != kd * g->n_point || g->n_node <= 0)
         return fail(H2P_HIP_EINVAL, "inconsistent sizes");
     if ((long long)g->n_point * kd > 0x7fffffffLL) return fail(H2P_HIP_EINVAL, "too many points");
+    if (g->periodic) {
+        if (mode != H2P_HIP_JIT || krnl->id != KR || prec != H2P_HIP_FP64)
+            return fail(H2P_HIP_ENOSYS, "the periodic mode is FP64 JIT with the RPY kernel (as H2P_build_periodic)");
+        if (!g->per_blk || (g->n_r_adm_pair > 0 && !g->per_adm_shifts) || (g->n_r_inadm_pair > 0 && !g->per_inadm_shifts))
+            return fail(H2P_HIP_EINVAL, "periodic representation without shifts or periodic block");
+    }
 
     h2p_hip_handle *h = new h2p_hip_handle();
     h->mode = mode;
@@ -535,10 +552,27 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     const int nD = nlf + g->n_r_inadm_pair;
     size_t dtot = g->D_ptr[nD];
     double B_pairs = 0, D_pairs = 0;
+    const bool per = g->periodic != 0;
+    h->periodic = per ? 1 : 0;
+    // shift table: entry 0 is the zero shift, then one entry per distinct offset
+    std::vector<double> shtab{0.0, 0.0, 0.0};
+    auto shift_index = [&](const double *v) {
+        if (v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0) return 0;
+        for (size_t k = 1; k < shtab.size() / 3; k++)
+            if (shtab[3 * k] == v[0] && shtab[3 * k + 1] == v[1] && shtab[3 * k + 2] == v[2]) return (int)k;
+        shtab.insert(shtab.end(), v, v + 3);
+        return (int)(shtab.size() / 3 - 1);
+    };
+    std::vector<int> adm_shift(g->n_r_adm_pair, 0), inadm_shift(g->n_r_inadm_pair, 0);
+    if (per) {
+        for (int i = 0; i < g->n_r_adm_pair; i++) adm_shift[i] = shift_index(g->per_adm_shifts + 3 * (size_t)i);
+        for (int i = 0; i < g->n_r_inadm_pair; i++) inadm_shift[i] = shift_index(g->per_inadm_shifts + 3 * (size_t)i);
+    }
     for (int i = 0; i < g->n_r_adm_pair; i++) {
         const int n0 = g->r_adm_pairs[2 * i], n1 = g->r_adm_pairs[2 * i + 1];
         const int l0 = g->node_level[n0], l1 = g->node_level[n1];
         BlkTask t{};
+        t.shift = adm_shift[i];
         t.nrow = g->B_nrow[i];
         t.ncol = g->B_ncol[i];
         if (l0 < l1) { t.c0 = g->pt_cluster[2 * n0]; t.n0 = npt(n0); t.in1 = g->mat_cluster[2 * n0]; }
@@ -552,11 +586,16 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         }
         if (t.nrow == 0 || t.ncol == 0) continue;
         t.data_off = (long long)g->B_ptr[i];
-        t.flags = 1;
         t.out0 = arena; arena += t.nrow;
-        t.out1 = arena; arena += t.ncol;
         (l0 < l1 ? dest_leaf[n0] : dest_y1[n0]).push_back(t.out0);
-        (l0 > l1 ? dest_leaf[n1] : dest_y1[n1]).push_back(t.out1);
+        if (per) {   // one-sided: node 0 receives only
+            t.flags = 0;
+            t.out1 = -1;
+        } else {
+            t.flags = 1;
+            t.out1 = arena; arena += t.ncol;
+            (l0 > l1 ? dest_leaf[n1] : dest_y1[n1]).push_back(t.out1);
+        }
         B_pairs += (double)t.n0 * t.n1;
         blk.push_back(t);
     }
@@ -576,7 +615,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         t.data_off = (long long)(btot + g->D_ptr[i]);
         t.out0 = arena; arena += t.nrow;
         dest_leaf[n0].push_back(t.out0);
-        if (i >= nlf) {
+        if (i >= nlf) t.shift = inadm_shift[i - nlf];
+        if (i >= nlf && !per) {
             t.flags = 1;
             t.out1 = arena; arena += t.ncol;
             dest_leaf[n1].push_back(t.out1);
@@ -601,7 +641,9 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             int n = g->level_nodes[(size_t)lv * nlf + j];
             const int p = parent[n];
             has_push[n] = (p >= 0 && g->node_level[p] >= g->min_adm_level && active[p] && g->n_child[p] > 0) ? 1 : 0;
-            active[n] = ((g->node_n_r_adm[n] > 0 || has_push[n]) && g->U_ncol[n] > 0) ? 1 : 0;
+            // periodic: the root's y1 is its periodic block product
+            const bool per_root = per && n == root;
+            active[n] = ((g->node_n_r_adm[n] > 0 || has_push[n] || per_root) && g->U_ncol[n] > 0) ? 1 : 0;
             if (!active[n]) has_push[n] = 0;
         }
     // ---- non-leaf skeleton sums (leaves sum their own in the output kernel)
@@ -684,7 +726,9 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             a.trans = 0; a.K = bc;
             a.c0 = set_start(n0, p0); a.c1 = set_start(n1, p1);
             a.x_space = p1 ? MM_PX : MM_Y0; a.x_off = p1 ? g->mat_cluster[2 * n1] : soff[n1];
+            a.shift = adm_shift[i];
             (p0 ? leaf_src[n0] : y1_src[n0]).push_back(a);
+            if (per) continue;   // one-sided pairs
             // entry of n1: op = B^T
             b.trans = 1; b.K = br;
             b.c0 = set_start(n1, p1); b.c1 = set_start(n0, p0);
@@ -704,8 +748,9 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             a.trans = 0; a.K = dc;
             a.c0 = g->pt_cluster[2 * n0]; a.c1 = g->pt_cluster[2 * n1];
             a.x_space = MM_PX; a.x_off = g->mat_cluster[2 * n1];
+            if (i >= nlf) a.shift = inadm_shift[i - nlf];
             leaf_src[n0].push_back(a);
-            if (i >= nlf) {
+            if (i >= nlf && !per) {
                 MMSrc b = a;
                 b.trans = 1; b.K = dr;
                 b.c0 = g->pt_cluster[2 * n1]; b.c1 = g->pt_cluster[2 * n0];
@@ -740,6 +785,14 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             }
             L.n = (int)mmt.size() - L.off;
             if (L.n > 0) h->mm_fwd_levels.push_back(L);
+        }
+        // periodic: the root's block, a dense source on its own skeleton
+        // coefficients (its only coupling: the root has no admissible pair)
+        if (per) {
+            MMSrc r{};
+            r.kind = MM_A_BD; r.a_off = 0; r.lda = g->U_ncol[root]; r.trans = 0; r.K = g->U_ncol[root];
+            r.x_space = MM_Y0; r.x_off = soff[root];
+            y1_src[root].push_back(r);
         }
         // coupling into Y1 (every node with a basis, so Y1 is always written)
         {
@@ -819,6 +872,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                       t.nrow == t.n0 * kd && t.ncol == t.n1 * kd;
             if (t.flags & 1) ok = ok && in(t.in1, t.nrow, XIN) && in(t.out1, t.ncol, AR);
             if (mode == H2P_HIP_AOT) ok = ok && in(t.data_off, (long long)t.nrow * t.ncol, BLOB);
+            ok = ok && t.shift >= 0 && t.shift < (int)(shtab.size() / 3);
             if (!ok) return bail(fail(H2P_HIP_EINVAL, "block task out of range"));
         }
         for (int i = 0; i < N; i++)
@@ -832,9 +886,11 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                 ok = x.x_space >= 0 && x.x_space < 4 && in(x.x_off, x.K, rows_of[x.x_space]) && x.K >= 0;
                 if (!ok) break;
                 if (x.kind == MM_A_JIT) {
-                    ok = (t.M % kd == 0) && (x.K % kd == 0) && in(x.c0, t.M / kd, CT) && in(x.c1, x.K / kd, CT);
+                    ok = (t.M % kd == 0) && (x.K % kd == 0) && in(x.c0, t.M / kd, CT) && in(x.c1, x.K / kd, CT) &&
+                         x.shift >= 0 && x.shift < (int)(shtab.size() / 3);
                 } else {
-                    const long long cap = (x.kind == MM_A_BD) ? BLOB : utot;
+                    const long long per_cap = (long long)g->U_ncol[root] * g->U_ncol[root];
+                    const long long cap = (x.kind == MM_A_BD) ? (per ? per_cap : BLOB) : utot;
                     const long long rows = x.trans ? x.K : t.M, cols = x.trans ? t.M : x.K;
                     ok = (x.kind == MM_A_U || x.kind == MM_A_UT || x.kind == MM_A_BD) && x.lda >= cols &&
                          (rows == 0 || in(x.a_off, (rows - 1) * (long long)x.lda + cols, cap));
@@ -843,12 +899,16 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             if (!ok) return bail(fail(H2P_HIP_EINVAL, "matmul task out of range"));
         }
     }
+    if (per && (g->n_child[root] == 0 || g->U_ncol[root] == 0))
+        return bail(fail(H2P_HIP_EINVAL, "periodic mode needs a root with children and a basis"));
+    const size_t per_elems = per ? (size_t)g->U_ncol[root] * g->U_ncol[root] : 0;
     if (dry) {
         auto vb = [](const auto &v) {
             return sizeof(typename std::decay_t<decltype(v)>::value_type) * std::max<size_t>(v.size(), 1);
         };
         size_t b = vb(cu) + vb(Ub) + vb(UTb) + sizeof(int) * std::max<size_t>(N, 1) + vb(blk) + vb(fwdt) +
-                   vb(bwdt) + vb(red) + vb(leaf) + vb(contrib) + vb(mmt) + vb(mms);
+                   vb(bwdt) + vb(red) + vb(leaf) + vb(contrib) + vb(mmt) + vb(mms) + vb(shtab) +
+                   (per ? sizeof(double) * per_elems : 0);
         b += sizeof(double) * (std::max<size_t>((size_t)N + S, 1) + 2 * std::max<size_t>(S, 1) +
                                std::max<size_t>((size_t)arena, 1));
         if (mode == H2P_HIP_AOT)
@@ -873,6 +933,13 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if ((rc = dev_upload(&h->d_contrib, contrib, acct))) return bail(rc);
     if ((rc = dev_upload(&h->d_mmt, mmt, acct))) return bail(rc);
     if ((rc = dev_upload(&h->d_mms, mms, acct))) return bail(rc);
+    if ((rc = dev_upload(&h->d_shift, shtab, acct))) return bail(rc);
+    if (per) {
+        std::vector<double> pv(g->per_blk, g->per_blk + per_elems);
+        if ((rc = dev_upload(&h->d_per, pv, acct))) return bail(rc);
+        h->per_r = g->U_ncol[root];
+        h->per_soff = soff[root];
+    }
     h->h_mmt = std::move(mmt);
     h->h_mms = std::move(mms);
     h->h_blk = blk;
@@ -1457,6 +1524,7 @@ static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const 
         c.blob = h->d_blob;
         c.fwd = h->d_fwd;
         c.contrib = h->d_contrib;
+        c.shifts = h->d_shift;
         c.N = h->N;
         c.pad = 0;
         c.x = x[i];
@@ -1660,6 +1728,7 @@ static int check_handles(int n, h2p_hip_handle *const *hs) {
     if (n > 0 && !hs) return fail(H2P_HIP_EINVAL, "NULL handle array");
     for (int i = 0; i < n; i++) {
         if (!hs[i]) return fail(H2P_HIP_EINVAL, "NULL handle");
+        if (hs[i]->periodic) return fail(H2P_HIP_ENOSYS, "periodic handles are not batched");
         for (int j = 0; j < i; j++)
             if (hs[j] == hs[i]) return fail(H2P_HIP_EINVAL, "a handle appears twice in one batch");
     }
@@ -1734,6 +1803,7 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
             MMCtx c{};
             c.nv = std::min(H2P_MM_MAXNVP, n_vec[i] - v0);
             c.srcs = h->d_mms;
+            c.shifts = h->d_shift;
             c.U = h->d_U;
             c.UT = h->d_UT;
             c.BD = (const double *)h->d_blob;

@@ -9,6 +9,7 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 
 #define KRNL_COULOMB  0
 #define KRNL_GAUSSIAN 1
@@ -146,4 +147,117 @@ void h2p_host_sketch(int kid, const double *param,
             for (int j = 0; j < n1; j++) out[(long)(j0 + j) * nx + c] = col[j];
         }
     }
+}
+
+/* Periodic RPY kernel by Ewald summation (the periodic system kernel of
+ * H2P_build_periodic), following RPY_Ewald_init_workbuf / RPY_Ewald_eval_std
+ * of src/H2Pack_3D_kernels.h:961-1210: for a cubic cell of side L, splitting
+ * parameter xi, real-space images |i| <= nr per axis and reciprocal vectors
+ * |k| <= nk per axis (half of them, by symmetry), each 3x3 block is the
+ * real-space sum (excluding r = 0) + the reciprocal sum + the overlap
+ * correction of the minimum image + the self term for coincident points.
+ * Coordinates: 4 rows (x, y, z, radius) of length ld.  Like the reference,
+ * the result carries no 1/(6 pi eta) factor. */
+int h2p_host_rpy_ewald(const double *c0, int ld0, int n0, const double *c1, int ld1, int n1,
+                       double L, double xi, int nr, int nk, double *mat, long ldm, int n_thread)
+{
+    const int r_size = (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
+    const int k_size = ((2 * nk + 1) * (2 * nk + 1) * (2 * nk + 1) - 1) / 2;
+    if (nr < 0 || nk < 0 || k_size > r_size) return -1;   /* the k vectors come from the nr cube */
+    double *rs = (double *)malloc(sizeof(double) * 3 * r_size);
+    double *kk = (double *)malloc(sizeof(double) * 6 * (k_size > 0 ? k_size : 1));
+    if (!rs || !kk) { free(rs); free(kk); return -2; }
+    const double V_inv = 1.0 / (L * L * L), tpl = 2.0 * M_PI / L;
+    /* one walk over the nr cube: every (ix, iy, iz) is a real-space shift,
+     * and the first k_size of them (lexicographic order, all before the
+     * centre when nk = nr) are the reciprocal vectors, as in
+     * RPY_Ewald_init_workbuf */
+    int idx = 0;
+    for (int ix = -nr; ix <= nr; ix++)
+        for (int iy = -nr; iy <= nr; iy++)
+            for (int iz = -nr; iz <= nr; iz++, idx++) {
+                rs[3 * idx] = L * ix; rs[3 * idx + 1] = L * iy; rs[3 * idx + 2] = L * iz;
+                if (idx < k_size) {
+                    const double k = tpl * sqrt((double)(ix * ix + iy * iy + iz * iz));
+                    const double k2 = k * k, q = k2 / (xi * xi);
+                    kk[6 * idx] = k;
+                    kk[6 * idx + 1] = 2.0 * V_inv * (1.0 + 0.25 * q + 0.125 * q * q) * 6.0 * M_PI / k2 * exp(-0.25 * q);
+                    kk[6 * idx + 2] = tpl * ix; kk[6 * idx + 3] = tpl * iy; kk[6 * idx + 4] = tpl * iz;
+                    kk[6 * idx + 5] = 1.0 / k;
+                }
+            }
+    const double xi2 = xi * xi, xi3 = xi2 * xi, xi5 = xi3 * xi2, xi7 = xi5 * xi2;
+    const double isp = 1.0 / sqrt(M_PI);
+    const double *x0 = c0, *y0 = c0 + ld0, *z0 = c0 + 2 * (long)ld0, *a0 = c0 + 3 * (long)ld0;
+    const double *x1 = c1, *y1 = c1 + ld1, *z1 = c1 + 2 * (long)ld1, *a1 = c1 + 3 * (long)ld1;
+    #pragma omp parallel for schedule(dynamic) num_threads(n_thread > 0 ? n_thread : 1)
+    for (int i = 0; i < n0; i++) {
+        const double ai = a0[i];
+        const double self_t = 1.0 / ai - (6.0 - 40.0 / 3.0 * xi2 * ai * ai) * xi * isp;
+        for (int j = 0; j < n1; j++) {
+            double m00 = 0, m10 = 0, m20 = 0, m11 = 0, m21 = 0, m22 = 0;
+            const double dx = x0[i] - x1[j], dy = y0[i] - y1[j], dz = z0[i] - z1[j], aj = a1[j];
+            const double a3 = 0.5 * (ai * ai + aj * aj);
+            for (int s = 0; s < r_size; s++) {               /* real-space sum */
+                double rx = dx + rs[3 * s], ry = dy + rs[3 * s + 1], rz = dz + rs[3 * s + 2];
+                const double r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 == 0.0) continue;
+                const double r = sqrt(r2), ri = 1.0 / r, ri2 = ri * ri, ri3 = ri * ri2, r4 = r2 * r2;
+                const double e = erfc(xi * r), g = isp * exp(-xi2 * r2);
+                const double t0 = 0.75 * ri, t1 = 0.5 * ri3 * a3, t2 = 4.0 * xi7 * a3 * r4;
+                const double t3 = 3.0 * xi3 * r2, t4 = 4.0 * xi5 * a3 * r2, t5 = 2.0 * xi3 * a3;
+                const double t6 = xi * a3 * ri2;
+                const double p = (t0 + t1) * e + (t2 + t3 - 5.0 * t4 - 4.5 * xi + 7.0 * t5 + t6) * g;
+                const double q = (t0 - 3.0 * t1) * e + (-t2 - t3 + 4.0 * t4 + 1.5 * xi - t5 - 3.0 * t6) * g;
+                rx *= ri; ry *= ri; rz *= ri;
+                m00 += q * rx * rx + p; m10 += q * rx * ry; m20 += q * rx * rz;
+                m11 += q * ry * ry + p; m21 += q * ry * rz; m22 += q * rz * rz + p;
+            }
+            for (int s = 0; s < k_size; s++) {               /* reciprocal sum */
+                const double k = kk[6 * s], m2 = kk[6 * s + 1], ki = kk[6 * s + 5];
+                const double t = m2 * cos(kk[6 * s + 2] * dx + kk[6 * s + 3] * dy + kk[6 * s + 4] * dz) *
+                                 (1.0 - a3 * k * k / 3.0);
+                const double ux = kk[6 * s + 2] * ki, uy = kk[6 * s + 3] * ki, uz = kk[6 * s + 4] * ki;
+                m00 += t * (1.0 - ux * ux); m10 -= t * ux * uy; m20 -= t * ux * uz;
+                m11 += t * (1.0 - uy * uy); m21 -= t * uy * uz; m22 += t * (1.0 - uz * uz);
+            }
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 >= 1e-30) {                               /* overlap correction, minimum image */
+                double rx = fmod(dx + 2 * L, L), ry = fmod(dy + 2 * L, L), rz = fmod(dz + 2 * L, L);
+                rx = rx > 0.5 * L ? rx - L : rx;
+                ry = ry > 0.5 * L ? ry - L : ry;
+                rz = rz > 0.5 * L ? rz - L : rz;
+                const double s2 = rx * rx + ry * ry + rz * rz, r = sqrt(s2), r3 = s2 * r;
+                const double ri = 1.0 / r, ri3 = ri * ri * ri;
+                rx *= ri; ry *= ri; rz *= ri;
+                const double u = (ai * ai + aj * aj) / s2;
+                const double f1 = 0.75 * ri * (1.0 + u / 3.0), f2 = 0.75 * ri * (1.0 - u);
+                const double d = ai - aj;
+                double t1, t2;
+                if (r > ai + aj) {
+                    t1 = f1; t2 = f2;
+                } else if (r > fabs(d)) {
+                    const double w = ri3 / (32.0 * ai * aj);
+                    t1 = d * d + 3.0 * s2;
+                    t1 = (16.0 * r3 * (ai + aj) - t1 * t1) * w;
+                    t2 = d * d - s2;
+                    t2 = 3.0 * t2 * t2 * w;
+                } else {
+                    t1 = 1.0 / (ai > aj ? ai : aj); t2 = 0.0;
+                }
+                t1 -= f1; t2 -= f2;
+                m00 += t2 * rx * rx + t1; m10 += t2 * rx * ry; m20 += t2 * rx * rz;
+                m11 += t2 * ry * ry + t1; m21 += t2 * ry * rz; m22 += t2 * rz * rz + t1;
+            } else {                                         /* self term */
+                m00 += self_t; m11 += self_t; m22 += self_t;
+            }
+            double *b = mat + (long)(3 * i) * ldm + 3 * j;
+            b[0] = m00;       b[1] = m10;       b[2] = m20;
+            b[ldm] = m10;     b[ldm + 1] = m11; b[ldm + 2] = m21;
+            b[2 * ldm] = m20; b[2 * ldm + 1] = m21; b[2 * ldm + 2] = m22;
+        }
+    }
+    free(rs);
+    free(kk);
+    return 0;
 }

@@ -21,7 +21,8 @@ struct BlkTask {
     int n0, n1;           // JIT: point counts
     int in0, in1;         // xin offsets of the operands (in0: ncol long, in1: nrow long)
     int flags;            // bit 0: has out1
-    int pad;
+    int shift;            // index into the shift table, added to set 1's coordinates
+                          // (periodic images; entry 0 is the zero shift)
 };
 
 // Upward-sweep task: y0[out] = U^T in, U row-major nrow x r.
@@ -343,11 +344,14 @@ __device__ __forceinline__ void blk_jit_pass(const BlkTask &t, int pbase, int np
                                              const double *__restrict__ cx,
                                              long long cstride,
                                              const KParam kp,
+                                             const double *__restrict__ shifts,
                                              const double *__restrict__ xin,
                                              double *__restrict__ arena,
                                              double (*red)[KD][4][H2P_WAVE], bool first) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+    // set 1 may be a periodic image: its coordinates plus the task's shift
+    const double hx = shifts[3 * t.shift], hy = shifts[3 * t.shift + 1], hz = shifts[3 * t.shift + 2];
     double sx[NS], sy[NS], sz[NS], sa[NS];
     double x0[NS][KD], acc1[NS][KD];
 #pragma unroll
@@ -355,7 +359,7 @@ __device__ __forceinline__ void blk_jit_pass(const BlkTask &t, int pbase, int np
         int p = k * H2P_WAVE + lane;
         bool ok = p < npts;
         int g = t.c1 + pbase + (ok ? p : 0);
-        sx[k] = cx[g]; sy[k] = cy[g]; sz[k] = cz[g];
+        sx[k] = cx[g] + hx; sy[k] = cy[g] + hy; sz[k] = cz[g] + hz;
         if constexpr (KID == KR) sa[k] = ca[g];
 #pragma unroll
         for (int b = 0; b < KD; b++) {
@@ -428,6 +432,7 @@ __device__ __forceinline__ void blk_jit_pass(const BlkTask &t, int pbase, int np
 template <int KID, int KD>
 __device__ __forceinline__ void blocks_jit_body(const BlkTask &t, const double *__restrict__ cx,
                                                 long long cstride, const KParam &kp,
+                                                const double *__restrict__ shifts,
                                                 const double *__restrict__ xin,
                                                 double *__restrict__ arena) {
     __shared__ double red[H2P_JIT_MAXNS][KD][4][H2P_WAVE];
@@ -435,9 +440,9 @@ __device__ __forceinline__ void blocks_jit_body(const BlkTask &t, const double *
         const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
         const int ns = (np + 63) >> 6;
         const bool first = (pb == 0);
-        if (ns <= 1) blk_jit_pass<KID, KD, 1>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
-        else if (ns <= 2) blk_jit_pass<KID, KD, 2>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
-        else blk_jit_pass<KID, KD, 4>(t, pb, np, cx, cstride, kp, xin, arena, red, first);
+        if (ns <= 1) blk_jit_pass<KID, KD, 1>(t, pb, np, cx, cstride, kp, shifts, xin, arena, red, first);
+        else if (ns <= 2) blk_jit_pass<KID, KD, 2>(t, pb, np, cx, cstride, kp, shifts, xin, arena, red, first);
+        else blk_jit_pass<KID, KD, 4>(t, pb, np, cx, cstride, kp, shifts, xin, arena, red, first);
         __syncthreads();
     }
 }
@@ -446,9 +451,25 @@ template <int KID, int KD>
 __global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ tasks,
                                                     const double *__restrict__ cx,
                                                     long long cstride, KParam kp,
+                                                    const double *__restrict__ shifts,
                                                     const double *__restrict__ xin,
                                                     double *__restrict__ arena) {
-    blocks_jit_body<KID, KD>(tasks[blockIdx.x], cx, cstride, kp, xin, arena);
+    blocks_jit_body<KID, KD>(tasks[blockIdx.x], cx, cstride, kp, shifts, xin, arena);
+}
+
+// The root's periodic block (H2P_matvec_periodic, src/H2Pack_matvec_periodic.c:
+// 455-470): y1[root] = per_blk y0[root], one wave per row, lanes over the
+// columns, fixed-order wave reduction.
+__global__ void __launch_bounds__(256) k_root_block(const double *__restrict__ per, int r,
+                                                    const double *__restrict__ y0,
+                                                    double *__restrict__ y1) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= r) return;
+    const double *a = per + (long long)row * r;
+    double s = 0.0;
+    for (int c = lane; c < r; c += H2P_WAVE) s += a[c] * y0[c];
+    s = wave_sum(s);
+    if (lane == 0) y1[row] = s;
 }
 
 // ---------------------------------------------------------------------------

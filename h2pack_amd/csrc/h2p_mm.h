@@ -33,6 +33,8 @@ struct MMSrc {
     int K;             // inner dimension (vector entries)
     int c0, c1;        // JIT: first destination / source point (unified coordinates)
     int x_space, x_off;
+    int shift;         // JIT: shift-table entry added to the source points (periodic images)
+    int pad;
 };
 
 struct MMTask {
@@ -79,7 +81,8 @@ __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict
                                         const double *__restrict__ UTblob,
                                         const double *__restrict__ BDblob,
                                         const double *__restrict__ cx, long long cstride,
-                                        const KParam &kp, const MMPanels &P) {
+                                        const KParam &kp, const double *__restrict__ shifts,
+                                        const MMPanels &P) {
     constexpr int RC = G::RC, KC = G::KC, NV = G::NV;
     constexpr int MT = RC / 16;                         // M tiles per chunk
     constexpr int SLOTS = (MT * (NV / 16) + 3) / 4;     // output tiles per wave, at most
@@ -103,13 +106,16 @@ __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict
                 // ---- A tile (RC x KC), zero outside the block
                 if (s.kind == MM_A_JIT) {
                     constexpr int RP = RC / KD, CP = KC / KD;
+                    const double hx = shifts[3 * s.shift], hy = shifts[3 * s.shift + 1],
+                                 hz = shifts[3 * s.shift + 2];
                     const int p0 = mc / KD, q0 = kc / KD;
                     const int np0 = t.M / KD - p0, nq0 = s.K / KD - q0;
                     for (int e = threadIdx.x; e < RP * CP; e += blockDim.x) {
                         const int i = e / CP, j = e - i * CP;
                         const bool ok = i < np0 && j < nq0;
                         const int gi = s.c0 + p0 + (ok ? i : 0), gj = s.c1 + q0 + (ok ? j : 0);
-                        const double dx = cx[gi] - cx[gj], dy = cy[gi] - cy[gj], dz = cz[gi] - cz[gj];
+                        const double dx = cx[gi] - (cx[gj] + hx), dy = cy[gi] - (cy[gj] + hy),
+                                     dz = cz[gi] - (cz[gj] + hz);
                         if constexpr (KD == 1) {
                             As[i][j] = ok ? kval<KID>(dx * dx + dy * dy + dz * dz, kp) : 0.0;
                         } else {
@@ -185,8 +191,8 @@ __global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
                                             const double *__restrict__ UTblob,
                                             const double *__restrict__ BDblob,
                                             const double *__restrict__ cx, long long cstride,
-                                            KParam kp, MMPanels P) {
-    mm_body<KID, KD, G>(tasks[blockIdx.x], srcs, Ublob, UTblob, BDblob, cx, cstride, kp, P);
+                                            KParam kp, const double *__restrict__ shifts, MMPanels P) {
+    mm_body<KID, KD, G>(tasks[blockIdx.x], srcs, Ublob, UTblob, BDblob, cx, cstride, kp, shifts, P);
 }
 
 // Caller panel (original order, row- or column-major, ld) -> PX (permuted,

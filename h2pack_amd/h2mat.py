@@ -85,6 +85,14 @@ def describe(h2, with_blocks: bool = True):
     else:
         d.B_data = None
         d.D_data = None
+    if getattr(h2, "is_RPY_Ewald", 0):
+        d.periodic = 1
+        d.per_adm_shifts = arr(np.asarray(h2.per_adm_shifts).reshape(-1), np.float64)
+        d.per_inadm_shifts = arr(np.asarray(h2.per_inadm_shifts).reshape(-1), np.float64)
+        d.per_blk = arr(h2.per_blk, np.float64)
+    else:
+        d.periodic = 0
+        d.per_adm_shifts = d.per_inadm_shifts = d.per_blk = None
     return d, keep
 
 

@@ -94,6 +94,16 @@ typedef struct {
     const int *D_nrow, *D_ncol;        /* n_leaf_node + n_r_inadm_pair     */
     const size_t *D_ptr;               /* n_leaf_node + n_r_inadm_pair + 1 */
     const double *D_data;              /* NULL: evaluated on the device    */
+    /* Periodic mode (H2P_run_RPY_Ewald + H2P_build_periodic, JIT RPY only;
+     * H2P_matvec_periodic, src/H2Pack_matvec_periodic.c:394).  Pairs are
+     * one-sided (node 0 receives), node 1's coordinates are shifted by the
+     * pair's lattice offset, and the root carries a dense periodic block
+     * applied to its skeleton coefficients.  periodic = 0: the fields below
+     * are ignored. */
+    int periodic;
+    const double *per_adm_shifts;      /* n_r_adm_pair x pt_dim            */
+    const double *per_inadm_shifts;    /* n_r_inadm_pair x pt_dim          */
+    const double *per_blk;             /* U_ncol[root]^2, row-major        */
 } h2p_hip_h2pack;
 
 typedef struct h2p_hip_handle h2p_hip_handle;

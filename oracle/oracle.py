@@ -219,3 +219,71 @@ def pcg(matvec, b: np.ndarray, tol: float, max_iter: int, x0: np.ndarray | None 
     flag = 0 if r_2norm <= rn_stop else 1                # pcg.c:125-127
     relres = r_2norm / b_2norm if b_2norm > 0 else float("nan")   # C's 0 / 0
     return x, flag, relres, it
+
+
+def periodic_matvec(h2, x: np.ndarray) -> np.ndarray:
+    """H2P_matvec_periodic (src/H2Pack_matvec_periodic.c:394-525), JIT, in
+    numpy over the oracle's kernel evaluation: forward permutation, upward
+    sweep, the one-sided shifted pair products in their three level cases
+    (:140-255), the root's periodic block y1[root] = per_blk y0[root]
+    (:455-470), downward sweep, the one-sided shifted dense products
+    (:300-370), backward permutation."""
+    kid, par, kd = h2.krnl_id, list(h2.krnl_param), h2.krnl_dim
+    nn, nl = h2.n_node, h2.n_leaf_node
+    ch = h2.children.reshape(nn, h2.max_child)
+    pc = h2.pt_cluster.reshape(-1, 2)
+    mc = h2.mat_cluster.reshape(-1, 2)
+    lvl = h2.node_level
+    px = np.asarray(x, dtype=np.float64)[h2.fwd_pmt_idx]
+    rows = lambda n: slice(mc[n, 0], mc[n, 1] + 1)                       # noqa: E731
+    pts = lambda n: h2.coord[:, pc[n, 0]:pc[n, 1] + 1]                      # noqa: E731
+
+    def shifted(c, s):
+        c = np.array(c, dtype=np.float64)
+        c[:len(s)] += np.asarray(s)[:, None]
+        return c
+
+    # upward sweep (H2P_matvec_fwd_transform)
+    y0 = [np.zeros(0) for _ in range(nn)]
+    for lv in range(h2.max_level, h2.min_adm_level - 1, -1):
+        for n in h2.level_nodes[lv * nl: lv * nl + h2.level_n_node[lv]]:
+            if h2.U_ncol[n] == 0:
+                continue
+            if h2.n_child[n] == 0:
+                y0[n] = h2.U[n].T @ px[rows(n)]
+            else:
+                y0[n] = h2.U[n].T @ np.concatenate([y0[c] for c in ch[n, :h2.n_child[n]]])
+    y1 = [np.zeros(h2.U_ncol[n]) for n in range(nn)]
+    py = np.zeros_like(px)
+    for i, (n0, n1) in enumerate(h2.r_adm_pairs.reshape(-1, 2)):
+        s = h2.per_adm_shifts[i]
+        if lvl[n0] == lvl[n1]:
+            y1[n0] += kernel_eval(kid, par, h2.J_coord[n0], shifted(h2.J_coord[n1], s)) @ y0[n1]
+        elif lvl[n0] > lvl[n1]:
+            y1[n0] += kernel_eval(kid, par, h2.J_coord[n0], shifted(pts(n1), s)) @ px[rows(n1)]
+        else:
+            py[rows(n0)] += kernel_eval(kid, par, pts(n0), shifted(h2.J_coord[n1], s)) @ y0[n1]
+    root = h2.root_idx
+    y1[root] = h2.per_blk @ y0[root]
+    # downward sweep (H2P_matvec_bwd_transform)
+    for lv in range(h2.min_adm_level, h2.max_level + 1):
+        for n in h2.level_nodes[lv * nl: lv * nl + h2.level_n_node[lv]]:
+            if h2.U_ncol[n] == 0:
+                continue
+            v = h2.U[n] @ y1[n]
+            if h2.n_child[n] == 0:
+                py[rows(n)] += v
+            else:
+                off = 0
+                for c in ch[n, :h2.n_child[n]]:
+                    y1[c] = y1[c] + v[off:off + h2.U_ncol[c]]
+                    off += h2.U_ncol[c]
+    # dense blocks: the leaves, then the shifted inadmissible pairs
+    for n in h2.leaf_nodes:
+        py[rows(n)] += kernel_eval(kid, par, pts(n), pts(n)) @ px[rows(n)]
+    for i, (n0, n1) in enumerate(h2.r_inadm_pairs.reshape(-1, 2)):
+        s = h2.per_inadm_shifts[i]
+        py[rows(n0)] += kernel_eval(kid, par, pts(n0), shifted(pts(n1), s)) @ px[rows(n1)]
+    y = np.empty_like(py)
+    y[h2.fwd_pmt_idx] = py
+    return y

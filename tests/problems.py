@@ -44,6 +44,18 @@ def problem(name: str, seed: int = 1234):
     return h2, coord
 
 
+@functools.lru_cache(maxsize=None)
+def periodic_problem(n: int = 500, L: float = 14.0, seed: int = 2):
+    """Periodic RPY (radius 0.5, eta = 1/(6 pi) so that the plain and Ewald
+    kernels share their scale) in a cubic cell of side L; Ewald parameters
+    xi = 0.3, 6 real-space and reciprocal layers; reltol 1e-6, leaf 60."""
+    rng = np.random.default_rng(seed)
+    coord = np.concatenate([rng.random((3, n)) * L, np.full((1, n), 0.5)], axis=0)
+    ewald = (L, 0.3, 6, 6)
+    h2 = C.build_periodic(coord, [0.0, 0.0, 0.0, L, L, L], 1.0 / (6.0 * np.pi), ewald, 1e-6, 60)
+    return h2, coord, ewald
+
+
 def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 

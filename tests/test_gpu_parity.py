@@ -220,3 +220,23 @@ def test_hss_mode_matches_oracle(name, mode):
           f"fp32mix {err32:.3e}", flush=True)
     assert err < FP64_TOL and err_mm < FP64_TOL
     assert err32 < 1e-6
+
+
+def test_periodic_matvec_and_matmul_match_oracle():
+    """Periodic RPY (H2P_matvec_periodic): one-sided shifted pairs and the
+    root's periodic block on the device, against the oracle's restatement
+    to 1e-12; matmul columns against the matvec."""
+    from tests.problems import periodic_problem
+    h2, _, _ = periodic_problem()
+    x = np.random.default_rng(12).standard_normal(h2.krnl_mat_size)
+    y_ref = O.periodic_matvec(h2, x)
+    m = H2MatHIP(h2, mode="jit")
+    err = rel(m.matvec(x), y_ref)
+    X = np.random.default_rng(13).standard_normal((h2.krnl_mat_size, 6))
+    Y = m.matmul(X, "col")
+    err_mm = max(rel(Y[:, j], O.periodic_matvec(h2, X[:, j])) for j in range(6))
+    print(f"[periodic rpy] rel err vs oracle: matvec {err:.3e}, matmul {err_mm:.3e}", flush=True)
+    assert err < FP64_TOL and err_mm < FP64_TOL
+    with pytest.raises(H2PHipError) as e:
+        H2MatHIP(h2, mode="aot")
+    assert e.value.code == abi.ENOSYS

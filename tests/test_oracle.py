@@ -195,3 +195,57 @@ def test_hss_representation_accuracy(name):
     err = rel(y, K @ x)
     print(f"[{name}] HSS rel err vs dense = {err:.2e}")
     assert err < 5e-4
+
+
+# ---------------------------------------------------------------------------
+# Periodic mode: the Ewald kernel (src/H2Pack_3D_kernels.h:961-1210) and the
+# periodic matvec restatement (oracle.periodic_matvec)
+def test_ewald_kernel_invariants():
+    """The Ewald sum does not depend on its splitting parameter once both
+    sums have converged; it is periodic in the cell and symmetric."""
+    rng = np.random.default_rng(1)
+    L = 12.0
+    c = np.concatenate([rng.random((3, 40)) * L, np.full((1, 40), 0.5)])
+    K1 = C.rpy_ewald_eval(c, c, L, 0.2, 4, 4)
+    K2 = C.rpy_ewald_eval(c, c, L, 0.3, 6, 6)
+    assert np.abs(K1 - K2).max() / np.abs(K1).max() < 1e-13
+    assert np.abs(K1 - K1.T).max() < 1e-14
+    c2 = c.copy(); c2[1] += L                      # every source moved one cell up
+    K3 = C.rpy_ewald_eval(c, c2, L, 0.3, 6, 6)
+    off = np.kron(~np.eye(40, dtype=bool), np.ones((3, 3), dtype=bool))
+    assert np.abs(K3 - K2)[off].max() / np.abs(K2).max() < 1e-13
+    # a planted mistake: truncated reciprocal sum breaks the xi independence
+    K4 = C.rpy_ewald_eval(c, c, L, 0.3, 6, 2)
+    assert np.abs(K4 - K2).max() / np.abs(K2).max() > 1e-6
+
+
+def test_periodic_pairs_are_one_sided_images():
+    from tests.problems import periodic_problem
+    h2, _, _ = periodic_problem()
+    assert h2.min_adm_level == 0 and h2.is_RPY_Ewald == 1
+    for pairs, shifts in ((h2.r_adm_pairs, h2.per_adm_shifts), (h2.r_inadm_pairs, h2.per_inadm_shifts)):
+        keys = {(int(a), int(b), tuple(np.round(s, 9))) for (a, b), s in zip(pairs.reshape(-1, 2), shifts)}
+        assert len(keys) == pairs.size // 2
+        for a, b, s in keys:                   # every interaction is present in both directions
+            assert (b, a, tuple(-v + 0.0 for v in s)) in keys
+    # B_p2i / D_p2i hold positive entries only (src/H2Pack_build.c:745-752)
+    assert np.all(h2.B_p2i_val > 0) and np.all(h2.D_p2i_val > 0)
+
+
+def test_periodic_oracle_against_dense_ewald():
+    """The periodic representation applied by the oracle equals the dense
+    Ewald kernel matrix to the build tolerance; dropping the root's periodic
+    block (a plausible mistake) is caught."""
+    from tests.problems import periodic_problem
+    h2, coord, (L, xi, nr, nk) = periodic_problem()
+    x = np.random.default_rng(3).standard_normal(h2.krnl_mat_size)
+    K = C.rpy_ewald_eval(coord, coord, L, xi, nr, nk)
+    yd = K @ x
+    y = O.periodic_matvec(h2, x)
+    assert rel(y, yd) < 1e-5
+    saved = h2.per_blk
+    try:
+        h2.per_blk = np.zeros_like(saved)
+        assert rel(O.periodic_matvec(h2, x), yd) > 1e-2
+    finally:
+        h2.per_blk = saved
