@@ -67,7 +67,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--check-rows", type=int, default=1000)
     ap.add_argument("--workload", default="gaussian1m",
-                    choices=["gaussian1m", "coulomb80k", "rpy_matmul", "precision2m", "serving"],
+                    choices=["gaussian1m", "coulomb80k", "rpy_matmul", "precision2m", "serving", "periodic"],
                     help="gaussian1m: the headline (default); the others are the remaining "
                          "configurations, each printed as its own JSON line")
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: most requests on the GPU at once")
     ap.add_argument("--cg-max-iter", type=int, default=3000, help="serving: CG iteration cap")
     ap.add_argument("--build-workers", type=int, default=4, help="serving: host construction processes")
+    ap.add_argument("--periodic-n", type=int, default=100_000, help="periodic: particles")
+    ap.add_argument("--periodic-rows", type=int, default=10, help="periodic: dense Ewald check rows")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 2 GiB device-to-device copy first, the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")

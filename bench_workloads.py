@@ -8,6 +8,9 @@ rpy_matmul   RPY tensor kernel (a = 1, eta = 1), N = 300,000 particles
              with 32 FP64 columns ~ N(0, 1), column-major; JIT and AOT.
 precision2m  3D Gaussian (l = 1) and 3D Coulomb at N = 2,000,000, density 10,
              reltol 1e-6, leaf 300, JIT: FP64 vs FP32-mixed matvec.
+periodic     periodic RPY-Ewald (H2P_matvec_periodic), radius 1 at 20% volume
+             fraction, --periodic-n particles, FP64 JIT; error against rows of
+             the dense Ewald matrix.
 serving      Poisson arrivals of Matern-3/2 GP requests (lengthscale U(0.5, 2),
              nugget 1e-2, N log-uniform in [1e4, 4e5], density 10, reltol
              1e-6, leaf 300): half 8-RHS products, half CG solves to 1e-6;
@@ -234,6 +237,54 @@ def run(args):
         out.update({"metric": "H2 matmuls/s (RPY N=300k, 32 RHS, FP64)", "value": res["jit"]["matmuls_per_s"],
                     "unit": "matmuls/s", "dtype": "f64", "jit": res["jit"], "aot": res["aot"],
                     "rel_err_vs_direct_nbody_1000rows_col0": _rel(Yr, yref), "build_s": tb})
+
+    elif args.workload == "periodic":
+        # periodic RPY (H2P_matvec_periodic): radius 1, 20 % volume fraction,
+        # eta = 1/(6 pi), Ewald xi = 0.015 with 6 layers each way (converged
+        # to machine precision for this cell), reltol 1e-6, leaf 200
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        N = args.periodic_n
+        L = (N * (4.0 / 3.0) * math.pi / 0.2) ** (1.0 / 3.0)
+        rng = np.random.default_rng(args.seed)
+        coord = np.concatenate([rng.random((3, N)) * L, np.ones((1, N))], axis=0)
+        ewald = (L, 0.015, 6, 6)
+        t0 = time.perf_counter()
+        h2 = C.build_periodic(coord, [0.0, 0.0, 0.0, L, L, L], 1.0 / (6.0 * math.pi), ewald, 1e-6, 200)
+        tb = time.perf_counter() - t0
+        log(f"[bench] periodic RPY N={N}: built in {tb:.1f}s, levels {h2.max_level + 1}, adm {h2.n_r_adm_pair}, "
+            f"inadm {h2.n_r_inadm_pair}, root rank {h2.U_ncol[h2.root_idx]}")
+        Nk = h2.krnl_mat_size
+        x = rng.standard_normal(Nk)
+        xd = torch.from_numpy(x).to(dev); yd = torch.empty_like(xd)
+        m = H2MatHIP(h2, mode="jit")
+        dt = _time(lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr), K, W, sync)
+        y = yd.cpu().numpy()
+        log(f"[bench] periodic: {1 / dt:.2f} matvecs/s ({dt * 1e3:.2f} ms)")
+        # accuracy: rows of the dense Ewald matrix (host, OpenMP)
+        rows = np.sort(rng.choice(N, args.periodic_rows, replace=False))
+        t0 = time.perf_counter()
+        Kr = C.rpy_ewald_eval(coord[:, rows], coord, *ewald)
+        te = time.perf_counter() - t0
+        yr = Kr @ x
+        yg = y.reshape(N, 3)[rows].reshape(-1)
+        err = _rel(yg, yr)
+        log(f"[bench] periodic: rel err vs dense Ewald on {rows.size} rows {err:.2e} ({te:.1f}s)")
+        cpu = None
+        if not args.no_cpu:
+            t0 = time.perf_counter()
+            yo = O.periodic_matvec(h2, x)
+            tc = time.perf_counter() - t0
+            cpu = {"value": 1.0 / tc, "unit": "matvecs/s", "cores": os.cpu_count() and min(16, os.cpu_count()),
+                   "kind": "port", "sample": "1 full matvec of the same representation by the oracle's periodic "
+                                             "restatement (numpy + the oracle's C kernel, BLAS threads)"}
+            log(f"[bench] periodic oracle: {tc:.1f}s per matvec; GPU vs oracle {_rel(y, yo):.2e}")
+        m.close()
+        out.update({"metric": "H2 matvecs/s (periodic RPY-Ewald, N=%d, FP64 JIT)" % N, "value": 1.0 / dt,
+                    "unit": "matvecs/s", "ms_per_step": dt * 1e3, "dtype": "f64",
+                    "rel_err_vs_dense_ewald_rows": err, "check_rows": int(rows.size), "build_s": tb,
+                    "cpu_baseline": cpu, "root_rank": int(h2.U_ncol[h2.root_idx]),
+                    "adm_pairs": h2.n_r_adm_pair, "inadm_pairs": h2.n_r_inadm_pair})
 
     else:   # precision2m
         N = 2_000_000
