@@ -75,6 +75,124 @@ template <int NV> struct MMVar<3, NV, 1> : MMCfg<96, 24, NV> {};
 template <int NV> struct MMVar<3, NV, 2> : MMCfg<96, 48, NV> {};
 template <int NV> struct MMVar<3, NV, 3> : MMCfg<48, 24, NV> {};
 
+// One stage of a task: the (RC x KC) operator tile of source si at inner
+// offset kc and the (KC x nvp) operand tile, fetched into registers (loads
+// or kernel evaluations), then stored to LDS.  The stages of a chunk run as
+// a software pipeline: stage s + 1 is fetched before the MFMAs of stage s
+// are issued, so its global-memory latency (or its kernel evaluations)
+// overlaps the matrix work.
+template <int KID, int KD, class G>
+struct MMStage {
+    static constexpr int RC = G::RC, KC = G::KC, NV = G::NV;
+    // per thread: NA kernel-evaluated pairs (JIT; 9 entries each for the 3x3
+    // kernel) or ND stored entries (dense), NX operand entries
+    static constexpr int NA = ((RC / KD) * (KC / KD) + 255) / 256;
+    static constexpr int ND = (RC * KC + 255) / 256;
+    static constexpr int NX = (KC * NV + 255) / 256;
+    static constexpr int NREG = (KD * KD * NA > ND) ? KD * KD * NA : ND;
+    double a[NREG];
+    double x[NX];
+
+    __device__ __forceinline__ void fetch(const MMTask &t, const MMSrc &s, int mc, int kc, int nvp,
+                                          const double *__restrict__ Ublob, const double *__restrict__ UTblob,
+                                          const double *__restrict__ BDblob, const double *__restrict__ cx,
+                                          long long cstride, const KParam &kp,
+                                          const double *__restrict__ shifts, const MMPanels &P) {
+        const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+        if (s.kind == MM_A_JIT) {
+            constexpr int RP = RC / KD, CP = KC / KD;
+            const int p0 = mc / KD, q0 = kc / KD;
+            const int np0 = t.M / KD - p0, nq0 = s.K / KD - q0;
+            const double hx = shifts[3 * s.shift], hy = shifts[3 * s.shift + 1], hz = shifts[3 * s.shift + 2];
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                const int e = threadIdx.x + 256 * u;
+                const int i = e / CP, j = e - i * CP;
+                const bool ok = e < RP * CP && i < np0 && j < nq0;
+                const int gi = s.c0 + p0 + (ok ? i : 0), gj = s.c1 + q0 + (ok ? j : 0);
+                const double dx = cx[gi] - (cx[gj] + hx), dy = cy[gi] - (cy[gj] + hy), dz = cz[gi] - (cz[gj] + hz);
+                if constexpr (KD == 1) {
+                    a[u] = ok ? kval<KID>(dx * dx + dy * dy + dz * dz, kp) : 0.0;
+                } else {
+                    double t1, t2, ux, uy, uz;
+                    rpy_coef(dx, dy, dz, ca[gi], ca[gj], kp.p1, t1, t2, ux, uy, uz);
+                    const double uu[3] = {ux, uy, uz};
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+#pragma unroll
+                        for (int q = 0; q < 3; q++)
+                            a[9 * u + 3 * p + q] = ok ? t2 * uu[p] * uu[q] + (p == q ? t1 : 0.0) : 0.0;
+                }
+            }
+        } else {
+            const double *A = (s.kind == MM_A_U ? Ublob : s.kind == MM_A_UT ? UTblob : BDblob) + s.a_off;
+#pragma unroll
+            for (int u = 0; u < ND; u++) {
+                const int e = threadIdx.x + 256 * u;
+                int i, j;
+                if (!s.trans) { i = e / KC; j = e - i * KC; }      // consecutive lanes along a row of A
+                else { j = e / RC; i = e - j * RC; }              // consecutive lanes along a row of A^T
+                const int m = mc + i, k = kc + j;
+                const bool ok = e < RC * KC && m < t.M && k < s.K;
+                a[u] = ok ? (s.trans ? A[(long long)k * s.lda + m] : A[(long long)m * s.lda + k]) : 0.0;
+            }
+        }
+        const double *X = P.p[s.x_space] + (long long)s.x_off * nvp;
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int e = threadIdx.x + 256 * u;
+            const int k = e / nvp, c = e - k * nvp;
+            x[u] = (e < KC * nvp && kc + k < s.K) ? X[(long long)(kc + k) * nvp + c] : 0.0;
+        }
+    }
+
+    __device__ __forceinline__ void store(const MMSrc &s, int nvp, double (*As)[KC + 1], double (*Xs)[NV + 1]) const {
+        if (s.kind == MM_A_JIT) {
+            constexpr int RP = RC / KD, CP = KC / KD;
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                const int e = threadIdx.x + 256 * u;
+                if (e >= RP * CP) break;
+                const int i = e / CP, j = e - i * CP;
+                if constexpr (KD == 1) {
+                    As[i][j] = a[u];
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+#pragma unroll
+                        for (int q = 0; q < 3; q++) As[3 * i + p][3 * j + q] = a[9 * u + 3 * p + q];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < ND; u++) {
+                const int e = threadIdx.x + 256 * u;
+                if (e >= RC * KC) break;
+                int i, j;
+                if (!s.trans) { i = e / KC; j = e - i * KC; }
+                else { j = e / RC; i = e - j * RC; }
+                As[i][j] = a[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int e = threadIdx.x + 256 * u;
+            if (e >= KC * nvp) break;
+            const int k = e / nvp, c = e - k * nvp;
+            Xs[k][c] = x[u];
+        }
+    }
+};
+
+// the stage after (si, kc): next inner chunk, or the first chunk of the next
+// source with K > 0
+__device__ __forceinline__ void mm_next(const MMSrc *__restrict__ srcs, int send, int KCs, int &si, int &kc) {
+    kc += KCs;
+    if (kc < srcs[si].K) return;
+    kc = 0;
+    for (si++; si < send && srcs[si].K <= 0; si++) {}
+}
+
 template <int KID, int KD, class G>
 __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict__ srcs,
                                         const double *__restrict__ Ublob,
@@ -93,76 +211,38 @@ __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lq = lane >> 4;
     double *dst = P.p[t.dst_space] + (long long)t.dst_off * nvp;
-    const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+    int s_first = t.sbeg;
+    while (s_first < t.send && srcs[s_first].K <= 0) s_first++;
+    MMStage<KID, KD, G> stg;
     for (int mc = 0; mc < t.M; mc += RC) {
         d4 acc[SLOTS];
 #pragma unroll
         for (int j = 0; j < SLOTS; j++) acc[j] = (d4){0.0, 0.0, 0.0, 0.0};
-        for (int si = t.sbeg; si < t.send; si++) {
-            const MMSrc s = srcs[si];
-            const double *X = P.p[s.x_space] + (long long)s.x_off * nvp;
-            for (int kc = 0; kc < s.K; kc += KC) {
-                __syncthreads();
-                // ---- A tile (RC x KC), zero outside the block
-                if (s.kind == MM_A_JIT) {
-                    constexpr int RP = RC / KD, CP = KC / KD;
-                    const double hx = shifts[3 * s.shift], hy = shifts[3 * s.shift + 1],
-                                 hz = shifts[3 * s.shift + 2];
-                    const int p0 = mc / KD, q0 = kc / KD;
-                    const int np0 = t.M / KD - p0, nq0 = s.K / KD - q0;
-                    for (int e = threadIdx.x; e < RP * CP; e += blockDim.x) {
-                        const int i = e / CP, j = e - i * CP;
-                        const bool ok = i < np0 && j < nq0;
-                        const int gi = s.c0 + p0 + (ok ? i : 0), gj = s.c1 + q0 + (ok ? j : 0);
-                        const double dx = cx[gi] - (cx[gj] + hx), dy = cy[gi] - (cy[gj] + hy),
-                                     dz = cz[gi] - (cz[gj] + hz);
-                        if constexpr (KD == 1) {
-                            As[i][j] = ok ? kval<KID>(dx * dx + dy * dy + dz * dz, kp) : 0.0;
-                        } else {
-                            double t1, t2, ux, uy, uz;
-                            rpy_coef(dx, dy, dz, ca[gi], ca[gj], kp.p1, t1, t2, ux, uy, uz);
-                            const double u[3] = {ux, uy, uz};
+        int si = s_first, kc = 0;
+        if (si < t.send)
+            stg.fetch(t, srcs[si], mc, kc, nvp, Ublob, UTblob, BDblob, cx, cstride, kp, shifts, P);
+        while (si < t.send) {
+            const MMSrc cur = srcs[si];
+            __syncthreads();                 // the previous stage's MFMAs are done with LDS
+            stg.store(cur, nvp, As, Xs);
+            __syncthreads();
+            int nsi = si, nkc = kc;
+            mm_next(srcs, t.send, KC, nsi, nkc);
+            if (nsi < t.send)                // next stage in flight during the MFMAs
+                stg.fetch(t, srcs[nsi], mc, nkc, nvp, Ublob, UTblob, BDblob, cx, cstride, kp, shifts, P);
 #pragma unroll
-                            for (int a = 0; a < 3; a++)
+            for (int j = 0; j < SLOTS; j++) {
+                const int tt = w + 4 * j;
+                if (tt < T) {
+                    const int mt = tt / NT, nt = tt - mt * NT;
 #pragma unroll
-                                for (int b = 0; b < 3; b++)
-                                    As[3 * i + a][3 * j + b] = ok ? t2 * u[a] * u[b] + (a == b ? t1 : 0.0) : 0.0;
-                        }
-                    }
-                } else {
-                    const double *A = (s.kind == MM_A_U ? Ublob : s.kind == MM_A_UT ? UTblob : BDblob) + s.a_off;
-                    if (!s.trans) {
-                        for (int e = threadIdx.x; e < RC * KC; e += blockDim.x) {
-                            const int i = e / KC, j = e - i * KC;
-                            const int m = mc + i, k = kc + j;
-                            As[i][j] = (m < t.M && k < s.K) ? A[(long long)m * s.lda + k] : 0.0;
-                        }
-                    } else {
-                        for (int e = threadIdx.x; e < RC * KC; e += blockDim.x) {
-                            const int j = e / RC, i = e - j * RC;
-                            const int m = mc + i, k = kc + j;
-                            As[i][j] = (m < t.M && k < s.K) ? A[(long long)k * s.lda + m] : 0.0;
-                        }
-                    }
-                }
-                // ---- X tile (KC x nvp), zero past K
-                for (int e = threadIdx.x; e < KC * nvp; e += blockDim.x) {
-                    const int k = e / nvp, c = e - k * nvp;
-                    Xs[k][c] = (kc + k < s.K) ? X[(long long)(kc + k) * nvp + c] : 0.0;
-                }
-                __syncthreads();
-#pragma unroll
-                for (int j = 0; j < SLOTS; j++) {
-                    const int tt = w + 4 * j;
-                    if (tt < T) {
-                        const int mt = tt / NT, nt = tt - mt * NT;
-#pragma unroll
-                        for (int kk = 0; kk < KC; kk += 4)
-                            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[mt * 16 + li][kk + lq],
-                                                                          Xs[kk + lq][nt * 16 + li], acc[j], 0, 0, 0);
-                    }
+                    for (int kk = 0; kk < KC; kk += 4)
+                        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[mt * 16 + li][kk + lq],
+                                                                      Xs[kk + lq][nt * 16 + li], acc[j], 0, 0, 0);
                 }
             }
+            si = nsi;
+            kc = nkc;
         }
         // ---- write the chunk: register r of lane (li, lq) holds row 4 r + lq
         // of its M tile, column li (the f64 16x16x4 output layout)
@@ -181,6 +261,7 @@ __device__ __forceinline__ void mm_body(const MMTask &t, const MMSrc *__restrict
                 }
             }
         }
+        __syncthreads();                     // LDS reuse by the next chunk
     }
 }
 
