@@ -23,9 +23,12 @@ HIP_SOURCES = ["h2p_hip.hip"]
 HIP_HEADERS = ["h2p_kernels.h", "h2p_device.h"]
 
 # Packed-FP32 instructions stay disabled: hipcc -O3 would otherwise form
-# v_pk_{add,mul,fma}_f32 from ordinary float code.
+# v_pk_{add,mul,fma}_f32 from ordinary float code.  MFMA results go to
+# ordinary VGPRs (-amdgpu-mfma-vgpr-form, device pass only): the kernels
+# that consume them on the VALU then need no accumulator-register reads.
 HIPCC_FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Xarch_device", "-mllvm=-amdgpu-mfma-vgpr-form",
                "-Wno-unused-result"]
 
 

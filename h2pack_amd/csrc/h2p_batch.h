@@ -30,7 +30,8 @@ struct ReqCtx {
     const int *fwd;
     const long long *contrib;
     const double *shifts;    // shift table (entry 0: zero)
-    int N, pad;
+    int N;
+    int nbig, ntask;         // grouped block launch: workgroup-unit tasks, all tasks
     // this call
     const double *x;
     double *y;
@@ -75,8 +76,8 @@ __global__ void __launch_bounds__(256) k_blocks_jit2_b(const BatchItem *__restri
                                                        const ReqCtx *__restrict__ reqs) {
     const int it = find_item(items, n, blockIdx.x);
     const ReqCtx &c = reqs[items[it].req];
-    const BlkTask t = ((const BlkTask *)items[it].tasks)[blockIdx.x - items[it].begin];
-    blocks_jit2_body<KID, T>(t, c.coord, c.CT, c.scale, c.diag, c.xin, c.arena);
+    blocks_jit2_unit<KID, T>((const BlkTask *)items[it].tasks, blockIdx.x - items[it].begin, c.nbig, c.ntask,
+                             c.coord, c.CT, c.scale, c.diag, c.xin, c.arena);
 }
 
 template <int KID, int KD>

@@ -117,6 +117,7 @@ struct h2p_hip_handle {
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
+    int n_big = 0, n_units = 0;   // grouped block launch: workgroup-unit tasks, launch units
     long long arena_len = 0;
     size_t device_bytes = 0;
     h2p_hip_info info{};
@@ -185,10 +186,11 @@ static void launch_build(h2p_hip_handle *h, hipStream_t s) {
 template <int KID, typename T>
 static void launch_jit2(h2p_hip_handle *h, hipStream_t s) {
     if (h->mv_var == 1)
-        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, sizeof(T) == 8 ? 4 : 5>), dim3(h->n_blk), dim3(256), 0, s,
-                           h->d_blk, h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, sizeof(T) == 8 ? 4 : 5>), dim3(h->n_units), dim3(256), 0, s,
+                           h->d_blk, h->n_big, h->n_blk, h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0,
+                           h->d_xin, h->d_arena);
     else
-        hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+        hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_units), dim3(256), 0, s, h->d_blk, h->n_big, h->n_blk,
                            h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
 }
 
@@ -634,6 +636,16 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         return (long long)a.nrow * a.ncol > (long long)b.nrow * b.ncol;
     });
     h->n_blk = (int)blk.size();
+    // FP64 norm-form JIT (Gaussian, Matern): the small blocks run as wave
+    // units of four, after the workgroup-unit blocks (each part keeps its
+    // longest-first order)
+    h->n_big = h->n_blk;
+    if (mode == H2P_HIP_JIT && prec == H2P_HIP_FP64 && (h->kid == KG || h->kid == KM)) {
+        auto big_end = std::stable_partition(blk.begin(), blk.end(),
+                                             [](const BlkTask &t) { return !jitm_wave_task(t.n0, t.n1); });
+        h->n_big = (int)(big_end - blk.begin());
+    }
+    h->n_units = h->n_big + (h->n_blk - h->n_big + 3) / 4;
     // ---- active nodes of the downward sweep (y1 "set", src/H2Pack_matvec.c:251-256, :802-838)
     std::vector<char> active(nn, 0), has_push(nn, 0);
     for (int lv = g->min_adm_level; lv <= g->max_level; lv++)
@@ -1526,7 +1538,8 @@ static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const 
         c.contrib = h->d_contrib;
         c.shifts = h->d_shift;
         c.N = h->N;
-        c.pad = 0;
+        c.nbig = h->n_big;
+        c.ntask = h->n_blk;
         c.x = x[i];
         c.y = y[i];
         c.shift = shift ? shift[i] : 0.0;
@@ -1570,7 +1583,7 @@ static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const 
             const h2p_hip_handle *a = hs[i], *b = hs[j];
             if (done[j] || a->mode != b->mode || a->prec != b->prec || a->kid != b->kid) continue;
             done[j] = 1;
-            L.add(j, b->d_blk, b->n_blk);
+            L.add(j, b->d_blk, b->n_units);
         }
         BLaunch bl = finish(tb, L);
         bl.mode = hs[i]->mode;

@@ -617,11 +617,225 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
     }
 }
 
-template <int KID, typename T>
-__device__ __forceinline__ void blocks_jit2_body(const BlkTask &t, const double *__restrict__ cx,
-                                                 long long cstride, double scale, double diag,
-                                                 const double *__restrict__ xin,
+// ---------------------------------------------------------------------------
+// FP64 norm-form kernels (Gaussian, Matern) with the distances on the matrix
+// cores.  The squared distances of a tile of 16 target rows x 16 source
+// columns are one v_mfma_f64_16x16x4f64,
+//     q[i][j] = |r_i|^2 + sum_k A[i][k] B[k][j],
+//     A[i] = (1, r_i),  B[.][j] = (|s_j|^2, -2 s_j),
+// with |r_i|^2 as the accumulator input (the same terms, in the same order,
+// as the VALU form |r|^2 + |s|^2 - 2 r.s), so the VALU only evaluates the
+// kernel and the two products of each pair.  MFMA layout, lane l: it
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
+// q[4 r + l / 16][l % 16] in register r.
+//  * A pass covers up to H2P_JITM_NT tiles of 16 source columns, staged in
+//    LDS (B operand and input value); each lane keeps one column partial
+//    sum per tile in registers.
+//  * Target rows are staged in LDS in chunks; the NW cooperating waves take
+//    the 16-row tiles w, w + NW, ...; per tile a lane accumulates four row
+//    sums (rows 4 r + l / 16) over its columns, reduced over the 16 lanes
+//    of each group once per tile.
+//  * Column partial sums are reduced at the end of the pass over the four
+//    lane groups (and, with NW = 4, the four waves) in a fixed order.
+//  * Padded rows repeat a valid row with input 0 and padded columns a valid
+//    source with input 0, so every kernel value is finite and the padding
+//    adds exact zeros.
+// Two launch units share one kernel: a workgroup unit (NW = 4) runs one
+// large block; a wave unit runs four small blocks (n0 <= H2P_JITW_ROWS,
+// n1 <= 16 H2P_JITM_NT), one per wave, with wave-private LDS and no
+// workgroup barriers, so the short blocks do not wait on each other.
+#define H2P_JITM_NT 8
+#define H2P_JITW_ROWS 80
+struct JitmRow { double a[4]; };   // (1, x, y, z) of a target; (|s|^2, -2 s) of a source
+struct JitmNV { double n, v; };    // |r|^2 and the transposed product's input
+
+// LDS of the norm-form kernels, in doubles: the workgroup unit uses rows
+// (4 + 2) x H2P_JIT_ROWS, sources 5 x 16 NT and the wave partials
+// 4 x NT x 64; a wave unit gives each wave (4 + 2) x H2P_JITW_ROWS + 5 x 16 NT.
+#define H2P_JITM_LDS_WG (6 * H2P_JIT_ROWS + 5 * 16 * H2P_JITM_NT + 4 * H2P_JITM_NT * H2P_WAVE)
+#define H2P_JITM_LDS_WAVE (6 * H2P_JITW_ROWS + 5 * 16 * H2P_JITM_NT)
+#define H2P_JITM_LDS (H2P_JITM_LDS_WG > 4 * H2P_JITM_LDS_WAVE ? H2P_JITM_LDS_WG : 4 * H2P_JITM_LDS_WAVE)
+
+// Orders the LDS accesses of one wave (LDS operations of a wave complete in
+// issue order; the fences keep the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NW>
+__device__ __forceinline__ void jitm_sync() {
+    if constexpr (NW == 4) __syncthreads();
+    else wave_lds_sync();
+}
+
+template <int KID, int NT, int NW>
+__device__ __forceinline__ void jitm_pass(const BlkTask &t, int pbase, int npts,
+                                          const double *__restrict__ cx, long long cstride,
+                                          double scale, double diag, double ox, double oy, double oz,
+                                          const double *__restrict__ xin, double *__restrict__ arena,
+                                          JitmRow *rowa, JitmNV *rownv, JitmRow *srcb, double *srcx,
+                                          double *red, bool first) {
+    constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
+    const int lane = threadIdx.x & 63, w = NW == 4 ? (threadIdx.x >> 6) : 0;
+    const int tid = NW == 4 ? threadIdx.x : lane;
+    const int li = lane & 15, lg = lane >> 4;
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride;
+    // the pass's sources: B operand (|s|^2, -2 s) and input value
+    for (int q = tid; q < NT * 16; q += NW * H2P_WAVE) {
+        const bool ok = q < npts;
+        const int g = t.c1 + pbase + (ok ? q : 0);
+        const double ux = (cx[g] - ox) * scale, uy = (cy[g] - oy) * scale, uz = (cz[g] - oz) * scale;
+        JitmRow b;
+        b.a[0] = ux * ux + uy * uy + uz * uz;
+        b.a[1] = -2.0 * ux;
+        b.a[2] = -2.0 * uy;
+        b.a[3] = -2.0 * uz;
+        srcb[q] = b;
+        srcx[q] = ok ? xin[t.in0 + pbase + q] : 0.0;
+    }
+    double acc[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) acc[c] = 0.0;
+    const bool has1 = t.flags & 1;
+    for (int rc = 0; rc < t.n0; rc += ROWS) {
+        const int nr = min(t.n0 - rc, ROWS);
+        const int ntile = (nr + 15) >> 4;
+        jitm_sync<NW>();
+        for (int q = tid; q < ntile * 16; q += NW * H2P_WAVE) {
+            const bool ok = q < nr;
+            const int g = t.c0 + rc + (ok ? q : 0);
+            const double ux = (cx[g] - ox) * scale, uy = (cy[g] - oy) * scale, uz = (cz[g] - oz) * scale;
+            JitmRow ra;
+            ra.a[0] = 1.0;
+            ra.a[1] = ux;
+            ra.a[2] = uy;
+            ra.a[3] = uz;
+            rowa[q] = ra;
+            JitmNV e;
+            e.n = ux * ux + uy * uy + uz * uz;
+            e.v = (ok && has1) ? xin[t.in1 + rc + q] : 0.0;
+            rownv[q] = e;
+        }
+        jitm_sync<NW>();
+        for (int tt = w; tt < ntile; tt += NW) {
+            const double a = rowa[tt * 16 + li].a[lg];
+            d4 cn;
+            double v[4], p[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const JitmNV e = rownv[tt * 16 + 4 * r + lg];
+                cn[r] = e.n;
+                v[r] = e.v;
+                p[r] = 0.0;
+            }
+            // the scheduling barriers keep the compiler from hoisting all NT
+            // products ahead of the VALU work (each live result costs 8
+            // registers); other waves cover the product's latency
+#pragma unroll
+            for (int c = 0; c < NT; c++) {
+                const double xc = srcx[c * 16 + li];
+                const d4 q = __builtin_amdgcn_mfma_f64_16x16x4f64(a, srcb[c * 16 + li].a[lg], cn, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const double kv = kprep<KID>(q[r], diag);
+                    p[r] += kv * xc;
+                    acc[c] += kv * v[r];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const double s = group16_sum4(p[0], p[1], p[2], p[3]);
+            const int myrow = tt * 16 + 4 * ((lane >> 2) & 3) + lg;
+            if ((lane & 3) == 0 && myrow < nr) {
+                double *o = arena + t.out0 + rc + myrow;
+                if (first) *o = s;
+                else *o += s;
+            }
+        }
+    }
+    if (!has1) return;
+    if constexpr (NW == 4) {
+#pragma unroll
+        for (int c = 0; c < NT; c++) red[(w * NT + c) * H2P_WAVE + lane] = acc[c];
+        __syncthreads();
+        for (int j = threadIdx.x; j < npts; j += blockDim.x) {
+            const int c = j >> 4, m = j & 15;
+            double s = 0.0;
+#pragma unroll
+            for (int w2 = 0; w2 < 4; w2++)
+#pragma unroll
+                for (int g = 0; g < 4; g++) s += red[(w2 * NT + c) * H2P_WAVE + g * 16 + m];
+            arena[t.out1 + pbase + j] = s;
+        }
+    } else {
+        // (g0 + g1) + (g2 + g3) over the four lane groups
+#pragma unroll
+        for (int c = 0; c < NT; c++) {
+            double s = acc[c] + __shfl_xor(acc[c], 16, H2P_WAVE);
+            s += __shfl_xor(s, 32, H2P_WAVE);
+            const int j = c * 16 + li;
+            if (lg == 0 && j < npts) arena[t.out1 + pbase + j] = s;
+        }
+    }
+}
+
+template <int KID, int NW>
+__device__ __forceinline__ void jitm_task(const BlkTask &t, const double *__restrict__ cx,
+                                          long long cstride, double scale, double diag,
+                                          const double *__restrict__ xin, double *__restrict__ arena,
+                                          double *lds) {
+    constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
+    JitmRow *rowa = (JitmRow *)lds;
+    JitmNV *rownv = (JitmNV *)(lds + 4 * ROWS);
+    JitmRow *srcb = (JitmRow *)(lds + 6 * ROWS);
+    double *srcx = lds + 6 * ROWS + 4 * 16 * H2P_JITM_NT;
+    double *red = srcx + 16 * H2P_JITM_NT;   // workgroup unit only
+    const double ox = cx[t.c0], oy = cx[cstride + t.c0], oz = cx[2 * cstride + t.c0];
+    for (int pb = 0; pb < t.n1; pb += H2P_JITM_NT * 16) {
+        const int np = min(t.n1 - pb, H2P_JITM_NT * 16);
+        const bool first = (pb == 0);
+        switch ((np + 15) >> 4) {
+#define H2P_JITM_CASE(n) \
+            case n: jitm_pass<KID, n, NW>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rowa, rownv, srcb, srcx, red, first); break;
+            H2P_JITM_CASE(1) H2P_JITM_CASE(2) H2P_JITM_CASE(3) H2P_JITM_CASE(4)
+            H2P_JITM_CASE(5) H2P_JITM_CASE(6) H2P_JITM_CASE(7) H2P_JITM_CASE(8)
+#undef H2P_JITM_CASE
+            default: break;
+        }
+        jitm_sync<NW>();
+    }
+}
+
+// Launch unit u of the norm-form launch: u < nbig runs task u on the whole
+// workgroup, later units run tasks nbig + 4 (u - nbig) + w, one per wave.
+template <int KID>
+__device__ __forceinline__ void blocks_jitm_unit(const BlkTask *__restrict__ tasks, int u, int nbig, int ntask,
+                                                 const double *__restrict__ cx, long long cstride,
+                                                 double scale, double diag, const double *__restrict__ xin,
                                                  double *__restrict__ arena) {
+    __shared__ double lds[H2P_JITM_LDS];
+    if (u < nbig) {
+        jitm_task<KID, 4>(tasks[u], cx, cstride, scale, diag, xin, arena, lds);
+    } else {
+        const int w = threadIdx.x >> 6;
+        const int i = nbig + 4 * (u - nbig) + w;
+        if (i < ntask)
+            jitm_task<KID, 1>(tasks[i], cx, cstride, scale, diag, xin, arena, lds + w * H2P_JITM_LDS_WAVE);
+    }
+}
+
+// Whether a task of the norm-form launch runs as a wave unit.
+__host__ __device__ inline bool jitm_wave_task(int n0, int n1) {
+    return n0 <= H2P_JITW_ROWS && n1 <= 16 * H2P_JITM_NT;
+}
+
+template <int KID, typename T>
+__device__ __forceinline__ void blocks_jit2_scalar(const BlkTask &t, const double *__restrict__ cx,
+                                                   long long cstride, double scale, double diag,
+                                                   const double *__restrict__ xin,
+                                                   double *__restrict__ arena) {
     __shared__ JitRow<T> rows[H2P_JIT_ROWS];
     __shared__ double red[H2P_JIT_MAXNS][4][H2P_WAVE];
     double ox = 0.0, oy = 0.0, oz = 0.0;
@@ -642,14 +856,30 @@ __device__ __forceinline__ void blocks_jit2_body(const BlkTask &t, const double 
     }
 }
 
+// Launch unit u of the grouped kernel-evaluating launch: the norm-form
+// kernels run workgroup and wave units, the others one task per workgroup
+// (nbig = ntask).
+template <int KID, typename T>
+__device__ __forceinline__ void blocks_jit2_unit(const BlkTask *__restrict__ tasks, int u, int nbig, int ntask,
+                                                 const double *__restrict__ cx, long long cstride,
+                                                 double scale, double diag, const double *__restrict__ xin,
+                                                 double *__restrict__ arena) {
+    if constexpr (JitNorm<KID, T>::value) {
+        blocks_jitm_unit<KID>(tasks, u, nbig, ntask, cx, cstride, scale, diag, xin, arena);
+    } else {
+        blocks_jit2_scalar<KID, T>(tasks[u], cx, cstride, scale, diag, xin, arena);
+    }
+}
+
 // Occupancy variant of the grouped launch (the autotuner's alternative for
 // the matvec blocks): the same body compiled for at least WPE waves per SIMD,
 // so with fewer registers per lane.  Same arithmetic, same results.
 template <int KID, typename T, int WPE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-k_blocks_jit2_occ(const BlkTask *__restrict__ tasks, const double *__restrict__ cx, long long cstride,
-                  double scale, double diag, const double *__restrict__ xin, double *__restrict__ arena) {
-    blocks_jit2_body<KID, T>(tasks[blockIdx.x], cx, cstride, scale, diag, xin, arena);
+k_blocks_jit2_occ(const BlkTask *__restrict__ tasks, int nbig, int ntask, const double *__restrict__ cx,
+                  long long cstride, double scale, double diag, const double *__restrict__ xin,
+                  double *__restrict__ arena) {
+    blocks_jit2_unit<KID, T>(tasks, blockIdx.x, nbig, ntask, cx, cstride, scale, diag, xin, arena);
 }
 
 template <typename TS, int WPE>
@@ -660,12 +890,12 @@ k_blocks_aot_occ(const BlkTask *__restrict__ tasks, const TS *__restrict__ blob,
 }
 
 template <int KID, typename T>
-__global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks,
+__global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks, int nbig, int ntask,
                                                      const double *__restrict__ cx,
                                                      long long cstride, double scale, double diag,
                                                      const double *__restrict__ xin,
                                                      double *__restrict__ arena) {
-    blocks_jit2_body<KID, T>(tasks[blockIdx.x], cx, cstride, scale, diag, xin, arena);
+    blocks_jit2_unit<KID, T>(tasks, blockIdx.x, nbig, ntask, cx, cstride, scale, diag, xin, arena);
 }
 
 // ---------------------------------------------------------------------------

@@ -47,8 +47,6 @@ struct MMPanels {
     int nvp;           // padded row length
 };
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-
 // Tile geometry of one variant: RC rows of the destination per chunk (a
 // multiple of 16, and of 48 for the 3x3 kernel so tiles hold whole points),
 // KC inner entries per LDS stage, NV the widest panel the variant takes.

@@ -13,6 +13,10 @@ SPECS = {
     "coulomb_diag": (C.KRNL_COULOMB, [2.5], 3000, 1.0, 100, 1e-6, "surface"),
     "gaussian": (C.KRNL_GAUSSIAN, [1.0], 6000, 10.0, 150, 1e-6, "numerical"),
     "matern32": (C.KRNL_MATERN32, [1.0], 6000, 10.0, 150, 1e-6, "numerical"),
+    # leaves of up to 300 points: blocks wider than one 128-column pass and
+    # taller than one 256-row chunk
+    "gaussian_leaf300": (C.KRNL_GAUSSIAN, [1.0], 17000, 10.0, 300, 1e-6, "numerical"),
+    "matern32_leaf300": (C.KRNL_MATERN32, [1.0], 17000, 10.0, 300, 1e-6, "numerical"),
     "rpy": (C.KRNL_RPY, [1.0], 2500, 0.03, 120, 1e-7, "surface"),
     # tiny trees for dense assembly
     "tiny_coulomb": (C.KRNL_COULOMB, [], 900, 1.0, 40, 1e-8, "surface"),
