@@ -299,6 +299,7 @@ def run(args):
             ys, ts = {}, {}
             for prec in ("fp64", "fp32mix"):
                 m = H2MatHIP(h2, mode="jit", prec=prec)
+                m.autotune(0)
                 ts[prec] = _time(lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr), K, W, sync)
                 ys[prec] = yd.cpu().numpy()
                 m.close()

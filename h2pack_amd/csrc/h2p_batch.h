@@ -71,9 +71,11 @@ __global__ void __launch_bounds__(256) k_fwd2_b(const BatchItem *__restrict__ it
     fwd2_body(t, c.U, c.xin);
 }
 
-template <int KID, typename T>
-__global__ void __launch_bounds__(256) k_blocks_jit2_b(const BatchItem *__restrict__ items, int n,
-                                                       const ReqCtx *__restrict__ reqs) {
+// WPE: waves per SIMD the kernel is compiled for (1: no constraint); the
+// tiled kernels run at 4, as the single-matrix launch's occupancy variant.
+template <int KID, typename T, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_blocks_jit2_b(const BatchItem *__restrict__ items, int n, const ReqCtx *__restrict__ reqs) {
     const int it = find_item(items, n, blockIdx.x);
     const ReqCtx &c = reqs[items[it].req];
     blocks_jit2_unit<KID, T>((const BlkTask *)items[it].tasks, blockIdx.x - items[it].begin, c.nbig, c.ntask,

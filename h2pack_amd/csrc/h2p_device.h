@@ -171,15 +171,16 @@ __device__ __forceinline__ T wave_sum4(T v0, T v1, T v2, T v3) {
 // Sums over the 16 lanes of each lane group (lanes 16 g .. 16 g + 15) of
 // four per-lane values at once.  On return the lanes with (l & 3) == 0 hold
 // the full sum of value (l >> 2) & 3 of their group.
-__device__ __forceinline__ double group16_sum4(double v0, double v1, double v2, double v3) {
+template <typename T>
+__device__ __forceinline__ T group16_sum4(T v0, T v1, T v2, T v3) {
     const int lane = threadIdx.x & 63;
     const bool b3 = lane & 8, b2 = lane & 4;
-    const double s0 = b3 ? v0 : v2, s1 = b3 ? v1 : v3;
-    const double k0 = b3 ? v2 : v0, k1 = b3 ? v3 : v1;
-    const double a0 = k0 + __shfl_xor(s0, 8, H2P_WAVE);
-    const double a1 = k1 + __shfl_xor(s1, 8, H2P_WAVE);
-    const double s = b2 ? a0 : a1, k = b2 ? a1 : a0;
-    double c = k + __shfl_xor(s, 4, H2P_WAVE);
+    const T s0 = b3 ? v0 : v2, s1 = b3 ? v1 : v3;
+    const T k0 = b3 ? v2 : v0, k1 = b3 ? v3 : v1;
+    const T a0 = k0 + __shfl_xor(s0, 8, H2P_WAVE);
+    const T a1 = k1 + __shfl_xor(s1, 8, H2P_WAVE);
+    const T s = b2 ? a0 : a1, k = b2 ? a1 : a0;
+    T c = k + __shfl_xor(s, 4, H2P_WAVE);
     c += __shfl_xor(c, 2, H2P_WAVE);
     c += __shfl_xor(c, 1, H2P_WAVE);
     return c;

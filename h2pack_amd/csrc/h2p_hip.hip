@@ -107,7 +107,7 @@ struct h2p_hip_handle {
     double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, panel_nvp wide
     int panel_nvp = 0;
     int mm_var[4][3] = {};        // tile variant per matmul phase and panel width class
-    int mv_var = 0;               // variant of the matvec block launch (0 default, 1 occupancy)
+    int mv_var = 0;               // variant of the matvec block launch (0 default, 1 other occupancy)
     // periodic mode: shift table (x, y, z per entry; entry 0 is zero) and the
     // root's periodic block (also the matmul's dense source for the root)
     int periodic = 0;
@@ -182,16 +182,17 @@ static void launch_build(h2p_hip_handle *h, hipStream_t s) {
                            h->d_coord, h->CT, h->kp, (double *)h->d_blob);
 }
 
-// occupancy of the variant-1 kernels: FP64 JIT 4 waves per SIMD, FP32 JIT 5
+// variant 0 (default): the kernel compiled for 4 waves per SIMD; variant 1:
+// compiled without an occupancy target (more registers, 2-3 waves per SIMD)
 template <int KID, typename T>
 static void launch_jit2(h2p_hip_handle *h, hipStream_t s) {
     if (h->mv_var == 1)
-        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, sizeof(T) == 8 ? 4 : 5>), dim3(h->n_units), dim3(256), 0, s,
-                           h->d_blk, h->n_big, h->n_blk, h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0,
-                           h->d_xin, h->d_arena);
-    else
         hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_units), dim3(256), 0, s, h->d_blk, h->n_big, h->n_blk,
                            h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+    else
+        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, 4>), dim3(h->n_units), dim3(256), 0, s,
+                           h->d_blk, h->n_big, h->n_blk, h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0,
+                           h->d_xin, h->d_arena);
 }
 
 static void dispatch_jit(h2p_hip_handle *h, hipStream_t s) {
@@ -636,11 +637,13 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         return (long long)a.nrow * a.ncol > (long long)b.nrow * b.ncol;
     });
     h->n_blk = (int)blk.size();
-    // FP64 norm-form JIT (Gaussian, Matern): the small blocks run as wave
-    // units of four, after the workgroup-unit blocks (each part keeps its
-    // longest-first order)
+    // tiled JIT kernels (FP64 Gaussian and Matern, FP32-mixed Coulomb,
+    // Gaussian and Matern): the small blocks run as wave units of four, after
+    // the workgroup-unit blocks (each part keeps its longest-first order)
     h->n_big = h->n_blk;
-    if (mode == H2P_HIP_JIT && prec == H2P_HIP_FP64 && (h->kid == KG || h->kid == KM)) {
+    const bool tiled = mode == H2P_HIP_JIT && h->kid != KR &&
+                       (prec == H2P_HIP_FP32MIX || h->kid == KG || h->kid == KM);
+    if (tiled) {
         auto big_end = std::stable_partition(blk.begin(), blk.end(),
                                              [](const BlkTask &t) { return !jitm_wave_task(t.n0, t.n1); });
         h->n_big = (int)(big_end - blk.begin());
@@ -1612,16 +1615,16 @@ static void launch_blocks_b(const BLaunch &b, const char *dbase, const ReqCtx *r
     }
     switch (b.kid) {
         case KC:
-            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KC, float>), grid, blk, 0, s, it, b.n_items, reqs);
-            else hipLaunchKernelGGL((k_blocks_jit2_b<KC, double>), grid, blk, 0, s, it, b.n_items, reqs);
+            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KC, float, 4>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KC, double, 1>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
         case KG:
-            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KG, float>), grid, blk, 0, s, it, b.n_items, reqs);
-            else hipLaunchKernelGGL((k_blocks_jit2_b<KG, double>), grid, blk, 0, s, it, b.n_items, reqs);
+            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KG, float, 4>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KG, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
         case KM:
-            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KM, float>), grid, blk, 0, s, it, b.n_items, reqs);
-            else hipLaunchKernelGGL((k_blocks_jit2_b<KM, double>), grid, blk, 0, s, it, b.n_items, reqs);
+            if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KM, float, 4>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KM, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
         default:
             hipLaunchKernelGGL((k_blocks_jit_b<KR, 3>), grid, blk, 0, s, it, b.n_items, reqs);

@@ -473,67 +473,35 @@ __global__ void __launch_bounds__(256) k_root_block(const double *__restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Grouped kernel-evaluating bi-matvec for the scalar kernels, FP64 (T =
-// double) or FP32-mixed (T = float).
-//
+// Grouped kernel-evaluating bi-matvec for FP64 Coulomb.  1/r has the
+// kernel's diagonal value at r = 0, so r = 0 must be exact: the distances
+// are formed from coordinate differences.
 //  * Source points (columns) live in lanes, NS strips of 64 per pass.
-//  * Target points (rows) are staged in LDS, already prepared (scaled, and
-//    for FP32 shifted to the block's first target so the FP32 coordinates are
-//    block-local); each wave takes four rows per iteration and reduces the
-//    four row sums with one butterfly.
+//  * Target points (rows) are staged in LDS; each wave takes four rows per
+//    iteration and reduces the four row sums with one butterfly.
 //  * Lanes without a source point use a valid point with x = 0, so every
 //    kernel value stays finite and no per-pair select is needed.
-//  * FP32 mode: kernel values and products in FP32; row sums leave each
-//    pass as FP32 and are folded into the FP64 partial outputs; the column
-//    sums are FP32 over tiles of 16 rows, folded into FP64 registers.
 #define H2P_JIT_ROWS 256                 // target rows staged per chunk
-// Staged target row: prepared coordinates, squared norm (norm-expanded
-// distances), and the input value the transposed product multiplies.
-template <typename T>
-struct JitRow { T x, y, z, n, v, pad; };
-template <>
-struct JitRow<float> { float x, y, z, v; };   // FP32 rows carry no norm
+struct JitRow { double x, y, z, v; };    // coordinates and the transposed product's input
 
-// FP64 Gaussian / Matern form the squared distance as |r|^2 + |s|^2 - 2 r.s
-// (one add + three FMA instead of three subtractions, a multiply and two
-// FMA) from block-local coordinates; Coulomb keeps differences so that r = 0
-// is detected exactly, and FP32 keeps differences for accuracy.
-template <int KID, typename T>
-struct JitNorm { static constexpr bool value = sizeof(T) == 8 && (KID == KG || KID == KM); };
-
-template <int KID, typename T, int NS>
-__device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
-                                          const double *__restrict__ cx, long long cstride,
-                                          double scale, double diag, double ox, double oy, double oz,
+template <int NS>
+__device__ __forceinline__ void jitc_pass(const BlkTask &t, int pbase, int npts,
+                                          const double *__restrict__ cx, long long cstride, double diag,
                                           const double *__restrict__ xin, double *__restrict__ arena,
-                                          JitRow<T> *rows, double (*red)[4][H2P_WAVE], bool first) {
-    constexpr bool NORM = JitNorm<KID, T>::value;
+                                          JitRow *rows, double (*red)[4][H2P_WAVE], bool first) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *cy = cx + cstride, *cz = cx + 2 * cstride;
-    const T dg = (T)diag;
-    T sx[NS], sy[NS], sz[NS], sn[NS], x0[NS];
-    double acc1d[NS];
-    T acc1[NS];
+    double sx[NS], sy[NS], sz[NS], x0[NS], acc1[NS];
 #pragma unroll
     for (int k = 0; k < NS; k++) {
         const int p = k * H2P_WAVE + lane;
         const bool ok = p < npts;
         const int g = t.c1 + pbase + (ok ? p : 0);
-        const double ux = (cx[g] - ox) * scale, uy = (cy[g] - oy) * scale, uz = (cz[g] - oz) * scale;
-        if constexpr (NORM) {
-            sx[k] = (T)(-2.0 * ux);
-            sy[k] = (T)(-2.0 * uy);
-            sz[k] = (T)(-2.0 * uz);
-            sn[k] = (T)(ux * ux + uy * uy + uz * uz);
-        } else {
-            sx[k] = (T)ux;
-            sy[k] = (T)uy;
-            sz[k] = (T)uz;
-            sn[k] = (T)0;
-        }
-        x0[k] = ok ? (T)xin[t.in0 + pbase + p] : (T)0;
-        acc1[k] = (T)0;
-        acc1d[k] = 0.0;
+        sx[k] = cx[g];
+        sy[k] = cy[g];
+        sz[k] = cz[g];
+        x0[k] = ok ? xin[t.in0 + pbase + p] : 0.0;
+        acc1[k] = 0.0;
     }
     const bool has1 = t.flags & 1;
     for (int rc = 0; rc < t.n0; rc += H2P_JIT_ROWS) {
@@ -543,77 +511,67 @@ __device__ __forceinline__ void jit2_pass(const BlkTask &t, int pbase, int npts,
         for (int q = threadIdx.x; q < nr_pad; q += blockDim.x) {
             const bool ok = q < nr;
             const int g = t.c0 + rc + (ok ? q : 0);
-            const double ux = (cx[g] - ox) * scale, uy = (cy[g] - oy) * scale, uz = (cz[g] - oz) * scale;
-            JitRow<T> r;
-            r.x = (T)ux;
-            r.y = (T)uy;
-            r.z = (T)uz;
-            if constexpr (NORM) {
-                r.n = (T)(ux * ux + uy * uy + uz * uz);
-                r.pad = (T)0;
-            }
-            r.v = (ok && has1) ? (T)xin[t.in1 + rc + q] : (T)0;
+            JitRow r;
+            r.x = cx[g];
+            r.y = cy[g];
+            r.z = cz[g];
+            r.v = (ok && has1) ? xin[t.in1 + rc + q] : 0.0;
             rows[q] = r;
         }
         __syncthreads();
-        int tile = 0;
         for (int r0 = 4 * w; r0 < nr_pad; r0 += 16) {
-            T p[4];
-            JitRow<T> rr[4];
+            double p[4];
+            JitRow rr[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
                 rr[a] = rows[r0 + a];
-                p[a] = (T)0;
+                p[a] = 0.0;
             }
 #pragma unroll
             for (int k = 0; k < NS; k++) {
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
-                    T q;
-                    if constexpr (NORM) {
-                        q = rr[a].n + sn[k];
-                        q = __builtin_fma(rr[a].x, sx[k], q);
-                        q = __builtin_fma(rr[a].y, sy[k], q);
-                        q = __builtin_fma(rr[a].z, sz[k], q);
-                    } else {
-                        const T dx = rr[a].x - sx[k], dy = rr[a].y - sy[k], dz = rr[a].z - sz[k];
-                        q = dx * dx + dy * dy + dz * dz;
-                    }
-                    const T kv = kprep<KID>(q, dg);
+                    const double dx = rr[a].x - sx[k], dy = rr[a].y - sy[k], dz = rr[a].z - sz[k];
+                    const double kv = kprep<KC>(dx * dx + dy * dy + dz * dz, diag);
                     p[a] += kv * x0[k];
                     acc1[k] += kv * rr[a].v;
                 }
             }
-            const T s = wave_sum4<T>(p[0], p[1], p[2], p[3]);
+            const double s = wave_sum4<double>(p[0], p[1], p[2], p[3]);
             const int myrow = r0 + ((lane >> 4) & 3);
             if ((lane & 15) == 0 && myrow < nr) {
                 double *o = arena + t.out0 + rc + myrow;
-                if (first) *o = (double)s;
-                else *o += (double)s;
-            }
-            if constexpr (sizeof(T) == 4) {
-                if (++tile == 4) {   // fold the FP32 column sums of 16 rows
-                    tile = 0;
-#pragma unroll
-                    for (int k = 0; k < NS; k++) {
-                        acc1d[k] += (double)acc1[k];
-                        acc1[k] = (T)0;
-                    }
-                }
+                if (first) *o = s;
+                else *o += s;
             }
         }
     }
-#pragma unroll
-    for (int k = 0; k < NS; k++) acc1d[k] += (double)acc1[k];
     if (has1) {
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < NS; k++) red[k][w][lane] = acc1d[k];
+        for (int k = 0; k < NS; k++) red[k][w][lane] = acc1[k];
         __syncthreads();
         for (int c = threadIdx.x; c < npts; c += blockDim.x) {
             const int k = c >> 6, l = c & 63;
             arena[t.out1 + pbase + c] = (red[k][0][l] + red[k][1][l]) + (red[k][2][l] + red[k][3][l]);
         }
+    }
+}
+
+__device__ __forceinline__ void blocks_jitc(const BlkTask &t, const double *__restrict__ cx, long long cstride,
+                                            double diag, const double *__restrict__ xin,
+                                            double *__restrict__ arena) {
+    __shared__ JitRow rows[H2P_JIT_ROWS];
+    __shared__ double red[H2P_JIT_MAXNS][4][H2P_WAVE];
+    for (int pb = 0; pb < t.n1; pb += H2P_JIT_MAXNS * H2P_WAVE) {
+        const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
+        const int ns = (np + 63) >> 6;
+        const bool first = (pb == 0);
+        if (ns <= 1) jitc_pass<1>(t, pb, np, cx, cstride, diag, xin, arena, rows, red, first);
+        else if (ns <= 2) jitc_pass<2>(t, pb, np, cx, cstride, diag, xin, arena, rows, red, first);
+        else if (ns <= 3) jitc_pass<3>(t, pb, np, cx, cstride, diag, xin, arena, rows, red, first);
+        else jitc_pass<4>(t, pb, np, cx, cstride, diag, xin, arena, rows, red, first);
+        __syncthreads();
     }
 }
 
@@ -826,48 +784,194 @@ __device__ __forceinline__ void blocks_jitm_unit(const BlkTask *__restrict__ tas
     }
 }
 
-// Whether a task of the norm-form launch runs as a wave unit.
+// ---------------------------------------------------------------------------
+// FP32-mixed scalar kernels (Coulomb, Gaussian, Matern) in the same tiling
+// and launch units, with the distances on the VALU from coordinate
+// differences (FP32 keeps differences for accuracy, and Coulomb's r = 0 stays
+// exact).  Lane l takes column l % 16 of a tile and rows 4 r + l / 16.
+// Kernel values and products in FP32; a pass's row sums (up to
+// 16 H2P_JITM_NT columns) leave it as FP32 into the FP64 partial outputs;
+// each lane's column sums are FP32 over one staged row chunk and are then
+// folded into FP64 registers.
+struct JitfPt { float x, y, z, v; };   // prepared coordinates and input value
+
+// LDS in doubles: workgroup unit rows 2 H2P_JIT_ROWS, sources 2 x 16 NT,
+// wave partials 4 x NT x 64; a wave unit 2 H2P_JITW_ROWS + 2 x 16 NT per wave.
+#define H2P_JITF_LDS_WG (2 * H2P_JIT_ROWS + 2 * 16 * H2P_JITM_NT + 4 * H2P_JITM_NT * H2P_WAVE)
+#define H2P_JITF_LDS_WAVE (2 * H2P_JITW_ROWS + 2 * 16 * H2P_JITM_NT)
+#define H2P_JITF_LDS (H2P_JITF_LDS_WG > 4 * H2P_JITF_LDS_WAVE ? H2P_JITF_LDS_WG : 4 * H2P_JITF_LDS_WAVE)
+
+template <int KID, int NT, int NW>
+__device__ __forceinline__ void jitf_pass(const BlkTask &t, int pbase, int npts,
+                                          const double *__restrict__ cx, long long cstride,
+                                          double scale, double diag, double ox, double oy, double oz,
+                                          const double *__restrict__ xin, double *__restrict__ arena,
+                                          JitfPt *rows, JitfPt *srcs, double *red, bool first) {
+    constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
+    const int lane = threadIdx.x & 63, w = NW == 4 ? (threadIdx.x >> 6) : 0;
+    const int tid = NW == 4 ? threadIdx.x : lane;
+    const int li = lane & 15, lg = lane >> 4;
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride;
+    const float dg = (float)diag;
+    for (int q = tid; q < NT * 16; q += NW * H2P_WAVE) {
+        const bool ok = q < npts;
+        const int g = t.c1 + pbase + (ok ? q : 0);
+        JitfPt e;
+        e.x = (float)((cx[g] - ox) * scale);
+        e.y = (float)((cy[g] - oy) * scale);
+        e.z = (float)((cz[g] - oz) * scale);
+        e.v = ok ? (float)xin[t.in0 + pbase + q] : 0.0f;
+        srcs[q] = e;
+    }
+    float acc[NT];
+    double accd[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) {
+        acc[c] = 0.0f;
+        accd[c] = 0.0;
+    }
+    const bool has1 = t.flags & 1;
+    for (int rc = 0; rc < t.n0; rc += ROWS) {
+        const int nr = min(t.n0 - rc, ROWS);
+        const int ntile = (nr + 15) >> 4;
+        jitm_sync<NW>();
+        for (int q = tid; q < ntile * 16; q += NW * H2P_WAVE) {
+            const bool ok = q < nr;
+            const int g = t.c0 + rc + (ok ? q : 0);
+            JitfPt e;
+            e.x = (float)((cx[g] - ox) * scale);
+            e.y = (float)((cy[g] - oy) * scale);
+            e.z = (float)((cz[g] - oz) * scale);
+            e.v = (ok && has1) ? (float)xin[t.in1 + rc + q] : 0.0f;
+            rows[q] = e;
+        }
+        jitm_sync<NW>();
+        for (int tt = w; tt < ntile; tt += NW) {
+            float rx[4], ry[4], rz[4], rv[4], p[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const JitfPt e = rows[tt * 16 + 4 * r + lg];
+                rx[r] = e.x;
+                ry[r] = e.y;
+                rz[r] = e.z;
+                rv[r] = e.v;
+                p[r] = 0.0f;
+            }
+            // one column tile at a time (the barriers bound the registers
+            // the scheduler may spend on interleaving tiles)
+#pragma unroll
+            for (int c = 0; c < NT; c++) {
+                const JitfPt e = srcs[c * 16 + li];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float dx = rx[r] - e.x, dy = ry[r] - e.y, dz = rz[r] - e.z;
+                    const float kv = kprep<KID>(dx * dx + dy * dy + dz * dz, dg);
+                    p[r] += kv * e.v;
+                    acc[c] += kv * rv[r];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const float s = group16_sum4(p[0], p[1], p[2], p[3]);
+            const int myrow = tt * 16 + 4 * ((lane >> 2) & 3) + lg;
+            if ((lane & 3) == 0 && myrow < nr) {
+                double *o = arena + t.out0 + rc + myrow;
+                if (first) *o = (double)s;
+                else *o += (double)s;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NT; c++) {   // fold the chunk's FP32 column sums
+            accd[c] += (double)acc[c];
+            acc[c] = 0.0f;
+        }
+    }
+    if (!has1) return;
+    if constexpr (NW == 4) {
+#pragma unroll
+        for (int c = 0; c < NT; c++) red[(w * NT + c) * H2P_WAVE + lane] = accd[c];
+        __syncthreads();
+        for (int j = threadIdx.x; j < npts; j += blockDim.x) {
+            const int c = j >> 4, m = j & 15;
+            double s = 0.0;
+#pragma unroll
+            for (int w2 = 0; w2 < 4; w2++)
+#pragma unroll
+                for (int g = 0; g < 4; g++) s += red[(w2 * NT + c) * H2P_WAVE + g * 16 + m];
+            arena[t.out1 + pbase + j] = s;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NT; c++) {
+            double s = accd[c] + __shfl_xor(accd[c], 16, H2P_WAVE);
+            s += __shfl_xor(s, 32, H2P_WAVE);
+            const int j = c * 16 + li;
+            if (lg == 0 && j < npts) arena[t.out1 + pbase + j] = s;
+        }
+    }
+}
+
+template <int KID, int NW>
+__device__ __forceinline__ void jitf_task(const BlkTask &t, const double *__restrict__ cx,
+                                          long long cstride, double scale, double diag,
+                                          const double *__restrict__ xin, double *__restrict__ arena,
+                                          double *lds) {
+    constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
+    JitfPt *rows = (JitfPt *)lds;
+    JitfPt *srcs = (JitfPt *)(lds + 2 * ROWS);
+    double *red = lds + 2 * ROWS + 2 * 16 * H2P_JITM_NT;   // workgroup unit only
+    const double ox = cx[t.c0], oy = cx[cstride + t.c0], oz = cx[2 * cstride + t.c0];
+    for (int pb = 0; pb < t.n1; pb += H2P_JITM_NT * 16) {
+        const int np = min(t.n1 - pb, H2P_JITM_NT * 16);
+        const bool first = (pb == 0);
+        switch ((np + 15) >> 4) {
+#define H2P_JITF_CASE(n) \
+            case n: jitf_pass<KID, n, NW>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, srcs, red, first); break;
+            H2P_JITF_CASE(1) H2P_JITF_CASE(2) H2P_JITF_CASE(3) H2P_JITF_CASE(4)
+            H2P_JITF_CASE(5) H2P_JITF_CASE(6) H2P_JITF_CASE(7) H2P_JITF_CASE(8)
+#undef H2P_JITF_CASE
+            default: break;
+        }
+        jitm_sync<NW>();
+    }
+}
+
+template <int KID>
+__device__ __forceinline__ void blocks_jitf_unit(const BlkTask *__restrict__ tasks, int u, int nbig, int ntask,
+                                                 const double *__restrict__ cx, long long cstride,
+                                                 double scale, double diag, const double *__restrict__ xin,
+                                                 double *__restrict__ arena) {
+    __shared__ double lds[H2P_JITF_LDS];
+    if (u < nbig) {
+        jitf_task<KID, 4>(tasks[u], cx, cstride, scale, diag, xin, arena, lds);
+    } else {
+        const int w = threadIdx.x >> 6;
+        const int i = nbig + 4 * (u - nbig) + w;
+        if (i < ntask)
+            jitf_task<KID, 1>(tasks[i], cx, cstride, scale, diag, xin, arena, lds + w * H2P_JITF_LDS_WAVE);
+    }
+}
+
+// Whether a task of the tiled launches runs as a wave unit.
 __host__ __device__ inline bool jitm_wave_task(int n0, int n1) {
     return n0 <= H2P_JITW_ROWS && n1 <= 16 * H2P_JITM_NT;
 }
 
-template <int KID, typename T>
-__device__ __forceinline__ void blocks_jit2_scalar(const BlkTask &t, const double *__restrict__ cx,
-                                                   long long cstride, double scale, double diag,
-                                                   const double *__restrict__ xin,
-                                                   double *__restrict__ arena) {
-    __shared__ JitRow<T> rows[H2P_JIT_ROWS];
-    __shared__ double red[H2P_JIT_MAXNS][4][H2P_WAVE];
-    double ox = 0.0, oy = 0.0, oz = 0.0;
-    if constexpr (sizeof(T) == 4 || JitNorm<KID, T>::value) {   // block-local origin
-        ox = cx[t.c0];
-        oy = cx[cstride + t.c0];
-        oz = cx[2 * cstride + t.c0];
-    }
-    for (int pb = 0; pb < t.n1; pb += H2P_JIT_MAXNS * H2P_WAVE) {
-        const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
-        const int ns = (np + 63) >> 6;
-        const bool first = (pb == 0);
-        if (ns <= 1) jit2_pass<KID, T, 1>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
-        else if (ns <= 2) jit2_pass<KID, T, 2>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
-        else if (ns <= 3) jit2_pass<KID, T, 3>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
-        else jit2_pass<KID, T, 4>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, red, first);
-        __syncthreads();
-    }
-}
-
-// Launch unit u of the grouped kernel-evaluating launch: the norm-form
-// kernels run workgroup and wave units, the others one task per workgroup
-// (nbig = ntask).
+// Launch unit u of the grouped kernel-evaluating launch: the FP64 norm-form
+// kernels and the FP32-mixed kernels run workgroup and wave units, FP64
+// Coulomb one task per workgroup (nbig = ntask).
 template <int KID, typename T>
 __device__ __forceinline__ void blocks_jit2_unit(const BlkTask *__restrict__ tasks, int u, int nbig, int ntask,
                                                  const double *__restrict__ cx, long long cstride,
                                                  double scale, double diag, const double *__restrict__ xin,
                                                  double *__restrict__ arena) {
-    if constexpr (JitNorm<KID, T>::value) {
+    if constexpr (sizeof(T) == 4) {
+        blocks_jitf_unit<KID>(tasks, u, nbig, ntask, cx, cstride, scale, diag, xin, arena);
+    } else if constexpr (KID == KG || KID == KM) {
         blocks_jitm_unit<KID>(tasks, u, nbig, ntask, cx, cstride, scale, diag, xin, arena);
     } else {
-        blocks_jit2_scalar<KID, T>(tasks[u], cx, cstride, scale, diag, xin, arena);
+        static_assert(KID == KC, "scalar kernels only");
+        blocks_jitc(tasks[u], cx, cstride, diag, xin, arena);
     }
 }
 

@@ -161,9 +161,11 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
  * has the same kernel, mode and block-size histogram.  Every variant
  * computes bitwise the same result.  Optional outputs: summed phase times
  * of the default variants and of the chosen ones, in milliseconds.
- * n_vec = 0 tunes the matvec instead: its grouped B/D launch has a default
- * and an occupancy variant (the same body compiled for more waves per SIMD,
- * with fewer registers), again with identical results.
+ * n_vec = 0 tunes the matvec instead: its grouped B/D launch has two
+ * builds of the same body compiled for different occupancies (JIT: 4 waves
+ * per SIMD by default, unconstrained registers as the alternative; AOT:
+ * unconstrained by default, 5 (FP64) or 8 (FP32) waves as the
+ * alternative), again with identical results.
  * (H2Pack has no counterpart: its products are fixed BLAS / SIMD loops.) */
 #define H2P_HIP_MM_VARIANTS 4
 int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms,

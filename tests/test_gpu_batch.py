@@ -124,10 +124,14 @@ def _shifted_oracle(h2, shift):
 
 
 @pytest.mark.parametrize("name,shift,max_iter", [("matern32", 1.0, 400), ("gaussian", 0.5, 400),
-                                                  ("matern32", 1e-2, 60)])
+                                                  ("matern32", 0.3, 20)])
 def test_cg_matches_oracle_pcg(name, shift, max_iter):
-    """Converged solves (shift 1.0 / 0.5) and one stopped by max_iter (the
-    nugget 1e-2 system needs several hundred iterations)."""
+    """Converged solves (shift 1.0 / 0.5) and one stopped by max_iter.  The
+    capped case is chosen where 20 iterations of CG amplify rounding little
+    (a 1e-15 relative perturbation of the operator moves the residual by
+    ~1e-15 and the iterate by ~3e-11); on the nugget 1e-2 system after 60
+    iterations the same perturbation moves the residual by ~1 %, so no
+    tight comparison with the oracle is possible there."""
     torch = _torch()
     h2, _ = problem(name)
     m = H2MatHIP(h2, mode="jit", prec="fp64")
@@ -148,14 +152,12 @@ def test_cg_matches_oracle_pcg(name, shift, max_iter):
     print(f"[{name}] CG iters gpu {its[0]} oracle {it_ref}; relres gpu {rr[0]:.3e} "
           f"oracle {relres_ref:.3e}; true residual {true_res:.3e}; "
           f"x rel diff {rel(x, x_ref):.3e}", flush=True)
-    if max_iter == 60:
-        # stopped by the cap on an ill-conditioned system (condition number
-        # ~1e4), where rounding differences grow along the iteration: same
-        # count, residuals within 5 %, iterates within 1e-3
+    if max_iter == 20:
+        # stopped by the cap: same count, residuals and iterates as the oracle's
         assert flag == 1 and it_ref == its[0] == max_iter
-        assert abs(rr[0] - relres_ref) <= 0.05 * relres_ref
-        assert abs(true_res - relres_ref) <= 0.05 * relres_ref
-        assert rel(x, x_ref) < 1e-3
+        assert abs(rr[0] - relres_ref) <= 1e-8 * relres_ref
+        assert abs(true_res - relres_ref) <= 1e-6 * relres_ref
+        assert rel(x, x_ref) < 1e-8
         return
     assert flag == 0 and it_ref < max_iter
     assert abs(its[0] - it_ref) <= 2
