@@ -272,12 +272,13 @@ def main():
                    "leaf": 300, "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / FP64_PEAK_TFLOPS,
-                     "kernel": ("k_blocks_jit2_occ<1, double, 4>" if tune_j["variants"][0] == 1
+                     "kernel": ("k_blocks_jit2_occ<1, double, 4>" if tune_j["variants"][0] == 0
                                 else "k_blocks_jit2<1, double>"), "kernel_ms": info_j["t_blocks"],
                      "flops_per_launch": flops, "block_variant": tune_j["variants"][0],
                      "traffic_source": "profiles/gaussian_1M_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "note": "achieved = H2Pack JIT_flops model (14 flops per Gaussian pair) / kernel time; "
-                             "FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak"},
+                             "FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak (one FP64 pipe: "
+                             "SQ_VALU_MFMA_COEXEC_CYCLES = 0)"},
         "cpu_baseline": cpu,
         "rel_err_vs_direct_nbody_1000rows": err_direct,
         "rel_err_vs_oracle": err_oracle,
