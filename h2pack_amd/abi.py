@@ -54,6 +54,7 @@ class Info(ct.Structure):
         ("t_fwd", ct.c_double), ("t_blocks", ct.c_double), ("t_reduce", ct.c_double),
         ("t_bwd", ct.c_double), ("t_final", ct.c_double), ("t_total", ct.c_double),
         ("n_profiled", ct.c_int), ("mv_variant", ct.c_int),
+        ("t_blocks_b", ct.c_double), ("t_blocks_d", ct.c_double),
     ]
 
     def as_dict(self):

@@ -30,8 +30,7 @@ struct ReqCtx {
     const int *fwd;
     const long long *contrib;
     const double *shifts;    // shift table (entry 0: zero)
-    int N;
-    int nbig, ntask;         // grouped block launch: workgroup-unit tasks, all tasks
+    int N, pad;
     // this call
     const double *x;
     double *y;
@@ -42,6 +41,8 @@ struct BatchItem {
     int begin;       // first workgroup of the item
     int req;         // index into the ReqCtx table
     const void *tasks;
+    int nbig, ntask; // grouped block launches: workgroup-unit tasks, all tasks of the item
+    int pad0, pad1;
 };
 
 // items[0..n) sorted by begin, items[n].begin = total workgroups
@@ -78,7 +79,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8
 k_blocks_jit2_b(const BatchItem *__restrict__ items, int n, const ReqCtx *__restrict__ reqs) {
     const int it = find_item(items, n, blockIdx.x);
     const ReqCtx &c = reqs[items[it].req];
-    blocks_jit2_unit<KID, T>((const BlkTask *)items[it].tasks, blockIdx.x - items[it].begin, c.nbig, c.ntask,
+    const BatchItem &I = items[it];
+    blocks_jit2_unit<KID, T>((const BlkTask *)I.tasks, blockIdx.x - I.begin, I.nbig, I.ntask,
                              c.coord, c.CT, c.scale, c.diag, c.xin, c.arena);
 }
 

@@ -74,6 +74,9 @@ inline bool fwdidx_ok(int v, int n) { return v >= 0 && v < n; }
 
 }  // namespace
 
+// profiling events recorded per matvec call (see launch_interior)
+#define H2P_EV_PER_CALL 7
+
 struct h2p_hip_handle {
     int mode = 0, prec = 0, kid = 0, kd = 1, xpt_dim = 3;
     int N = 0, n_point = 0, S = 0;
@@ -117,7 +120,14 @@ struct h2p_hip_handle {
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
-    int n_big = 0, n_units = 0;   // grouped block launch: workgroup-unit tasks, launch units
+    // grouped block launches: part 0 = B blocks (all blocks when not split),
+    // part 1 = D blocks; each part a contiguous range of d_blk with its
+    // workgroup-unit tasks first
+    struct BlkPart { int off = 0, n = 0, big = 0, units = 0; };
+    BlkPart part[2];
+    bool split = false;             // D blocks on aux_stream, concurrent with the sweeps
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     long long arena_len = 0;
     size_t device_bytes = 0;
     h2p_hip_info info{};
@@ -163,6 +173,9 @@ static void free_handle(h2p_hip_handle *h) {
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    if (h->aux_stream) (void)hipStreamDestroy(h->aux_stream);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     delete h;
 }
 
@@ -184,24 +197,34 @@ static void launch_build(h2p_hip_handle *h, hipStream_t s) {
 
 // variant 0 (default): the kernel compiled for 4 waves per SIMD; variant 1:
 // compiled without an occupancy target (more registers, 2-3 waves per SIMD)
-template <int KID, typename T>
-static void launch_jit2(h2p_hip_handle *h, hipStream_t s) {
+template <int KID, typename T, int PART>
+static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s) {
+    const h2p_hip_handle::BlkPart &P = h->part[PART];
+    if (P.n == 0) return;
+    const BlkTask *tasks = h->d_blk + P.off;
     if (h->mv_var == 1)
-        hipLaunchKernelGGL((k_blocks_jit2<KID, T>), dim3(h->n_units), dim3(256), 0, s, h->d_blk, h->n_big, h->n_blk,
+        hipLaunchKernelGGL((k_blocks_jit2<KID, T, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
                            h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
     else
-        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, 4>), dim3(h->n_units), dim3(256), 0, s,
-                           h->d_blk, h->n_big, h->n_blk, h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0,
-                           h->d_xin, h->d_arena);
+        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, 4, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
+                           h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
 }
 
-static void dispatch_jit(h2p_hip_handle *h, hipStream_t s) {
+template <int KID, typename T>
+static void launch_jit2(h2p_hip_handle *h, hipStream_t s, int part) {
+    if (part == 0) launch_jit2_part<KID, T, 0>(h, s);
+    else launch_jit2_part<KID, T, 1>(h, s);
+}
+
+static void dispatch_jit(h2p_hip_handle *h, hipStream_t s, int part) {
     const bool f32 = h->prec == H2P_HIP_FP32MIX;
     switch (h->kid) {
-        case KC: f32 ? launch_jit2<KC, float>(h, s) : launch_jit2<KC, double>(h, s); break;
-        case KG: f32 ? launch_jit2<KG, float>(h, s) : launch_jit2<KG, double>(h, s); break;
-        case KM: f32 ? launch_jit2<KM, float>(h, s) : launch_jit2<KM, double>(h, s); break;
-        default: launch_jit<KR, 3>(h, s); break;
+        case KC: f32 ? launch_jit2<KC, float>(h, s, part) : launch_jit2<KC, double>(h, s, part); break;
+        case KG: f32 ? launch_jit2<KG, float>(h, s, part) : launch_jit2<KG, double>(h, s, part); break;
+        case KM: f32 ? launch_jit2<KM, float>(h, s, part) : launch_jit2<KM, double>(h, s, part); break;
+        default:
+            if (part == 0) launch_jit<KR, 3>(h, s);
+            break;
     }
 }
 
@@ -214,9 +237,15 @@ static void dispatch_build(h2p_hip_handle *h, hipStream_t s) {
     }
 }
 
-// The grouped B/D launch of a matvec (variant 1: the occupancy kernels, AOT
-// FP64 at 5 waves per SIMD, AOT FP32 at 8).
-static void launch_blocks(h2p_hip_handle *h, hipStream_t s) {
+// Part `part` of the grouped B/D launch of a matvec (AOT variant 1: the
+// occupancy kernels, FP64 at 5 waves per SIMD, FP32 at 8).  AOT and RPY
+// handles are not split: part 0 holds every block.
+static void launch_blocks(h2p_hip_handle *h, hipStream_t s, int part) {
+    if (h->part[part].n == 0) return;
+    if (h->mode != H2P_HIP_AOT) {
+        dispatch_jit(h, s, part);
+        return;
+    }
     const dim3 g(h->n_blk), b(256);
     if (h->mode == H2P_HIP_AOT && h->prec == H2P_HIP_FP32MIX) {
         if (h->mv_var == 1)
@@ -232,28 +261,42 @@ static void launch_blocks(h2p_hip_handle *h, hipStream_t s) {
         else
             hipLaunchKernelGGL(k_blocks_aot<double>, g, b, 0, s, h->d_blk, (const double *)h->d_blob, h->d_xin,
                                h->d_arena);
-    } else {
-        dispatch_jit(h, s);
     }
 }
 
-// Interior of a matvec: upward sweep, grouped B/D launch, skeleton sums,
-// downward sweep (reads xin[0, N) = px, writes yp / y1 / arena).
+// Interior of a matvec: upward sweep, grouped B/D launches, skeleton sums,
+// downward sweep (reads xin[0, N) = px, writes yp / y1 / arena).  On a split
+// handle the D blocks, which read only px, run on aux_stream from the start
+// of the interior and are joined at its end, so their kernel evaluations
+// overlap the HBM-bound sweeps.  With profiling events (ev: 7 per call) every
+// phase runs in order on s instead: ev[1] after the upward sweep, ev[2]
+// after the B blocks, ev[3] after the D blocks, ev[4] after the skeleton
+// sums, ev[5] after the downward sweep.
 static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
+    const bool fork = h->split && !ev && h->part[1].n > 0;
+    if (fork) {
+        (void)hipEventRecord(h->ev_fork, s);
+        (void)hipStreamWaitEvent(h->aux_stream, h->ev_fork, 0);
+        launch_blocks(h, h->aux_stream, 1);
+        (void)hipEventRecord(h->ev_join, h->aux_stream);
+    }
     for (const Level &L : h->fwd_levels)
         hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
     if (ev) (void)hipEventRecord(ev[1], s);
-    if (h->n_blk > 0) launch_blocks(h, s);
+    launch_blocks(h, s, 0);
     if (ev) (void)hipEventRecord(ev[2], s);
+    if (!fork) launch_blocks(h, s, 1);
+    if (ev) (void)hipEventRecord(ev[3], s);
     if (h->n_red > 0)
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
     if (h->periodic)
         hipLaunchKernelGGL(k_root_block, dim3((h->per_r + 3) / 4), dim3(256), 0, s, h->d_per, h->per_r,
                            h->d_xin + h->N + h->per_soff, h->d_y1 + h->per_soff);
-    if (ev) (void)hipEventRecord(ev[3], s);
+    if (ev) (void)hipEventRecord(ev[4], s);
     for (const Level &L : h->bwd_levels)
         hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
-    if (ev) (void)hipEventRecord(ev[4], s);
+    if (ev) (void)hipEventRecord(ev[5], s);
+    if (fork) (void)hipStreamWaitEvent(s, h->ev_join, 0);
 }
 
 static int ensure_graph(h2p_hip_handle *h) {
@@ -281,7 +324,7 @@ static void drop_matvec_graph(h2p_hip_handle *h) {
 // One matvec on device pointers (x, y in the caller's order).
 static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
     const bool prof = h->profiling && h->ev_used < h->ev_cap;
-    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * 6] : nullptr;
+    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * H2P_EV_PER_CALL] : nullptr;
     if (prof) HIP_TRY(hipEventRecord(ev[0], s));
     hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
     if (!prof && h->use_graph) {
@@ -295,7 +338,7 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
                        h->d_UT, h->d_yp, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
     HIP_TRY(hipGetLastError());
     if (prof) {
-        HIP_TRY(hipEventRecord(ev[5], s));
+        HIP_TRY(hipEventRecord(ev[6], s));
         h->ev_used++;
     }
     return 0;
@@ -602,6 +645,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         B_pairs += (double)t.n0 * t.n1;
         blk.push_back(t);
     }
+    const int nB = (int)blk.size();
     for (int i = 0; i < nD; i++) {
         BlkTask t{};
         int n0, n1;
@@ -632,23 +676,32 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         blk.push_back(t);
     }
     h->arena_len = arena;
-    // longest-processing-time order for the grouped launch
-    std::stable_sort(blk.begin(), blk.end(), [](const BlkTask &a, const BlkTask &b) {
-        return (long long)a.nrow * a.ncol > (long long)b.nrow * b.ncol;
-    });
     h->n_blk = (int)blk.size();
-    // tiled JIT kernels (FP64 Gaussian and Matern, FP32-mixed Coulomb,
-    // Gaussian and Matern): the small blocks run as wave units of four, after
-    // the workgroup-unit blocks (each part keeps its longest-first order)
-    h->n_big = h->n_blk;
+    // JIT handles of the scalar kernels launch the B blocks and the D blocks
+    // separately (the D blocks read only the permuted input, so they overlap
+    // the sweeps); AOT and RPY handles keep one grouped launch.  Each part in
+    // longest-processing-time order; the tiled kernels (FP64 Gaussian and
+    // Matern, FP32-mixed Coulomb, Gaussian and Matern) then run the small
+    // blocks of a part as wave units of four, after its workgroup-unit blocks.
+    h->split = mode == H2P_HIP_JIT && h->kid != KR && !per;
     const bool tiled = mode == H2P_HIP_JIT && h->kid != KR &&
                        (prec == H2P_HIP_FP32MIX || h->kid == KG || h->kid == KM);
-    if (tiled) {
-        auto big_end = std::stable_partition(blk.begin(), blk.end(),
-                                             [](const BlkTask &t) { return !jitm_wave_task(t.n0, t.n1); });
-        h->n_big = (int)(big_end - blk.begin());
+    h->part[0] = h2p_hip_handle::BlkPart{};
+    h->part[1] = h2p_hip_handle::BlkPart{};
+    h->part[0].n = h->split ? nB : h->n_blk;
+    h->part[1].off = h->part[0].n;
+    h->part[1].n = h->n_blk - h->part[0].n;
+    for (auto &P : h->part) {
+        auto b0 = blk.begin() + P.off, b1 = b0 + P.n;
+        std::stable_sort(b0, b1, [](const BlkTask &x, const BlkTask &y) {
+            return (long long)x.nrow * x.ncol > (long long)y.nrow * y.ncol;
+        });
+        P.big = P.n;
+        if (tiled)
+            P.big = (int)(std::stable_partition(b0, b1, [](const BlkTask &t) { return !jitm_wave_task(t.n0, t.n1); }) -
+                          b0);
+        P.units = P.big + (P.n - P.big + 3) / 4;
     }
-    h->n_units = h->n_big + (h->n_blk - h->n_big + 3) / 4;
     // ---- active nodes of the downward sweep (y1 "set", src/H2Pack_matvec.c:251-256, :802-838)
     std::vector<char> active(nn, 0), has_push(nn, 0);
     for (int lv = g->min_adm_level; lv <= g->max_level; lv++)
@@ -1009,6 +1062,12 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         }
     }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "stream create failed"));
+    if (h->split) {
+        if (hipStreamCreate(&h->aux_stream) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(H2P_HIP_EDEVICE, "auxiliary stream / event create failed"));
+    }
     // the upload is synchronous: callers may use the handle on any stream,
     // including streams that do not order with the null stream
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "upload did not complete"));
@@ -1259,9 +1318,13 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
     int err = 0;
     for (int v = 0; v < 2 && !err; v++) {
         h->mv_var = v;
-        launch_blocks(h, s);   // warm-up
+        launch_blocks(h, s, 0);   // warm-up
+        launch_blocks(h, s, 1);
         if (hipEventRecord(e0, s) != hipSuccess) { err = 1; break; }
-        for (int r = 0; r < reps; r++) launch_blocks(h, s);
+        for (int r = 0; r < reps; r++) {
+            launch_blocks(h, s, 0);
+            launch_blocks(h, s, 1);
+        }
         float ms = 0.f;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
@@ -1372,7 +1435,7 @@ int h2p_hip_set_profiling(h2p_hip_handle *h, int on) {
     h->ev_used = 0;
     if (on && h->ev_cap == 0) {
         const int cap = 512;
-        h->ev_pool.resize((size_t)cap * 6);
+        h->ev_pool.resize((size_t)cap * H2P_EV_PER_CALL);
         for (auto &e : h->ev_pool)
             if (hipEventCreate(&e) != hipSuccess) return fail(H2P_HIP_EDEVICE, "hipEventCreate failed");
         h->ev_cap = cap;
@@ -1384,11 +1447,12 @@ int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info) {
     if (!h || !info) return fail(H2P_HIP_EINVAL, "NULL argument");
     if (h->ev_used > 0) {
         // average the phases of the calls profiled since the last read
-        double acc[5] = {0, 0, 0, 0, 0};
+        // intervals: fwd, B blocks, D blocks, skeleton sums, bwd, leaf outputs
+        double acc[H2P_EV_PER_CALL - 1] = {};
         for (int c = 0; c < h->ev_used; c++) {
-            hipEvent_t *ev = &h->ev_pool[(size_t)c * 6];
-            HIP_TRY(hipEventSynchronize(ev[5]));
-            for (int i = 0; i < 5; i++) {
+            hipEvent_t *ev = &h->ev_pool[(size_t)c * H2P_EV_PER_CALL];
+            HIP_TRY(hipEventSynchronize(ev[H2P_EV_PER_CALL - 1]));
+            for (int i = 0; i < H2P_EV_PER_CALL - 1; i++) {
                 float t = 0.f;
                 HIP_TRY(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
                 acc[i] += t;
@@ -1396,11 +1460,13 @@ int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info) {
         }
         const double n = h->ev_used;
         h->info.t_fwd = acc[0] / n;
-        h->info.t_blocks = acc[1] / n;
-        h->info.t_reduce = acc[2] / n;
-        h->info.t_bwd = acc[3] / n;
-        h->info.t_final = acc[4] / n;
-        h->info.t_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4]) / n;
+        h->info.t_blocks_b = acc[1] / n;
+        h->info.t_blocks_d = acc[2] / n;
+        h->info.t_blocks = (acc[1] + acc[2]) / n;
+        h->info.t_reduce = acc[3] / n;
+        h->info.t_bwd = acc[4] / n;
+        h->info.t_final = acc[5] / n;
+        h->info.t_total = (acc[0] + acc[1] + acc[2] + acc[3] + acc[4] + acc[5]) / n;
         h->info.n_profiled = h->ev_used;
         h->ev_used = 0;
     }
@@ -1491,12 +1557,16 @@ struct BLaunch {
 struct ItemList {
     std::vector<BatchItem> items;
     int total = 0;
-    void add(int req, const void *tasks, int count) {
+    // count workgroups; block launches also give the item's workgroup-unit
+    // tasks and its task count (wave units follow the workgroup units)
+    void add(int req, const void *tasks, int count, int nbig = 0, int ntask = 0) {
         if (count <= 0) return;
-        BatchItem it;
+        BatchItem it{};
         it.begin = total;
         it.req = req;
         it.tasks = tasks;
+        it.nbig = nbig;
+        it.ntask = ntask;
         items.push_back(it);
         total += count;
     }
@@ -1541,8 +1611,7 @@ static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const 
         c.contrib = h->d_contrib;
         c.shifts = h->d_shift;
         c.N = h->N;
-        c.nbig = h->n_big;
-        c.ntask = h->n_blk;
+        c.pad = 0;
         c.x = x[i];
         c.y = y[i];
         c.shift = shift ? shift[i] : 0.0;
@@ -1586,7 +1655,7 @@ static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const 
             const h2p_hip_handle *a = hs[i], *b = hs[j];
             if (done[j] || a->mode != b->mode || a->prec != b->prec || a->kid != b->kid) continue;
             done[j] = 1;
-            L.add(j, b->d_blk, b->n_units);
+            for (const auto &P : b->part) L.add(j, b->d_blk + P.off, P.units, P.big, P.n);
         }
         BLaunch bl = finish(tb, L);
         bl.mode = hs[i]->mode;

@@ -975,10 +975,11 @@ __device__ __forceinline__ void blocks_jit2_unit(const BlkTask *__restrict__ tas
     }
 }
 
-// Occupancy variant of the grouped launch (the autotuner's alternative for
-// the matvec blocks): the same body compiled for at least WPE waves per SIMD,
-// so with fewer registers per lane.  Same arithmetic, same results.
-template <int KID, typename T, int WPE>
+// Occupancy build of the grouped launch: the same body compiled for at
+// least WPE waves per SIMD, so with fewer registers per lane.  Same
+// arithmetic, same results.  PART (0: B blocks, 1: D blocks) only names the
+// launch, so that the two launches of a matvec are told apart in profiles.
+template <int KID, typename T, int WPE, int PART>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_blocks_jit2_occ(const BlkTask *__restrict__ tasks, int nbig, int ntask, const double *__restrict__ cx,
                   long long cstride, double scale, double diag, const double *__restrict__ xin,
@@ -993,7 +994,7 @@ k_blocks_aot_occ(const BlkTask *__restrict__ tasks, const TS *__restrict__ blob,
     blocks_aot_body<TS>(tasks[blockIdx.x], blob, xin, arena);
 }
 
-template <int KID, typename T>
+template <int KID, typename T, int PART>
 __global__ void __launch_bounds__(256) k_blocks_jit2(const BlkTask *__restrict__ tasks, int nbig, int ntask,
                                                      const double *__restrict__ cx,
                                                      long long cstride, double scale, double diag,

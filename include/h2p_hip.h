@@ -123,6 +123,10 @@ typedef struct {
     double t_fwd, t_blocks, t_reduce, t_bwd, t_final, t_total;
     int n_profiled;
     int mv_variant;      /* variant of the matvec block launch (h2p_hip_autotune) */
+    /* t_blocks split into its two launches on JIT handles of the scalar
+     * kernels (B blocks, D blocks; profiled calls run them in order);
+     * elsewhere t_blocks_b = t_blocks and t_blocks_d = 0 */
+    double t_blocks_b, t_blocks_d;
 } h2p_hip_info;
 
 /* Mirror an H2 representation into HBM.  mode = H2P_HIP_AOT / _JIT,

@@ -174,6 +174,37 @@ def test_graph_replays_match_eager_path(mode):
         assert np.array_equal(y_graph, y_eager), it
 
 
+@pytest.mark.parametrize("name,prec", [("gaussian", "fp64"), ("coulomb", "fp64"), ("matern32", "fp32mix")])
+def test_split_block_launches_agree(name, prec, monkeypatch):
+    """JIT handles launch the B blocks and the D blocks separately, the D
+    blocks on a second stream concurrent with the sweeps (a fork/join inside
+    the captured interior, or eager with graphs off).  Both must equal the
+    profiled path, which runs every launch in order on one stream, bit for
+    bit, and the profiled D launch must carry the near-field work."""
+    import torch
+    h2, _ = problem(name)
+    g = H2MatHIP(h2, mode="jit", prec=prec)
+    monkeypatch.setenv("H2P_HIP_NO_GRAPH", "1")
+    e = H2MatHIP(h2, mode="jit", prec=prec)
+    monkeypatch.delenv("H2P_HIP_NO_GRAPH")
+    rng = np.random.default_rng(23)
+    junk = torch.randn(1 << 20, device="cuda:0", dtype=torch.float64)
+    for it in range(3):
+        x = rng.random(h2.krnl_mat_size) - 0.5
+        y_graph = g.matvec(x)
+        junk.mul_(1.0001).add_(0.5)
+        y_eager = e.matvec(x)
+        torch.cuda.synchronize()
+        g.set_profiling(True)
+        y_serial = g.matvec(x)
+        info = g.info()
+        g.set_profiling(False)
+        assert np.array_equal(y_graph, y_serial), it
+        assert np.array_equal(y_eager, y_serial), it
+        assert info["t_blocks_d"] > 0.0 and info["t_blocks_b"] > 0.0
+        assert abs(info["t_blocks_b"] + info["t_blocks_d"] - info["t_blocks"]) < 1e-6 * max(1.0, info["t_blocks"])
+
+
 @pytest.mark.parametrize("name", ["gaussian", "rpy"])
 def test_matmul_graph_replays_match_eager_handle(name, monkeypatch):
     """Each matmul chunk's product phases are a captured hipGraph (one per
