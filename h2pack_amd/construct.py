@@ -594,7 +594,9 @@ def generate_proxy_point_ID(h2: H2Pack, kid: int, param, seed: int = 1112,
     pt_maxL = h2.root_enbox[pt_dim] * 0.5
     pt_minL = pt_maxL * 0.5 ** h2.max_level
     pt_maxL *= 0.25
-    num_pp = int(round(math.log2(pt_maxL / pt_minL))) + 1 if pt_maxL >= pt_minL else 0
+    # a zero-size root box (all points coincide, e.g. N = 1) has no
+    # admissible levels and needs no proxy points
+    num_pp = int(round(math.log2(pt_maxL / pt_minL))) + 1 if pt_maxL >= pt_minL > 0.0 else 0
     rng = np.random.default_rng(seed)
     pp = [np.zeros((h2.xpt_dim, 0)) for _ in range(n_level)]
     for i in range(num_pp):
