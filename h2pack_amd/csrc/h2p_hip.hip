@@ -826,13 +826,26 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                 leaf_src[n1].push_back(b);
             }
         }
+        // a destination of M rows becomes tasks of at most MM_PIECE rows (a
+        // multiple of every variant's row chunk, and of 3 for whole points),
+        // each with its sources restricted to its rows: more workgroups for
+        // the few large destinations of the upper levels, the same MFMA
+        // sequence per output element, one writer per row
+        const int MM_PIECE = kd == 3 ? 96 : 128;
         auto add_task = [&](int space, int off, int M, int accum, const std::vector<MMSrc> &src) {
-            MMTask t{};
-            t.dst_space = space; t.dst_off = off; t.M = M; t.accumulate = accum;
-            t.sbeg = (int)mms.size();
-            for (const MMSrc &x : src) mms.push_back(x);
-            t.send = (int)mms.size();
-            mmt.push_back(t);
+            for (int m0 = 0; m0 < M || (M == 0 && m0 == 0); m0 += MM_PIECE) {
+                MMTask t{};
+                t.dst_space = space; t.dst_off = off + m0; t.M = std::min(MM_PIECE, M - m0); t.accumulate = accum;
+                t.sbeg = (int)mms.size();
+                for (MMSrc x : src) {
+                    if (x.kind == MM_A_JIT) x.c0 += m0 / kd;
+                    else x.a_off += x.trans ? (long long)m0 : (long long)m0 * x.lda;
+                    mms.push_back(x);
+                }
+                t.send = (int)mms.size();
+                mmt.push_back(t);
+                if (M == 0) break;
+            }
         };
         auto work = [&](const MMTask &t) {
             long long wsum = 0;
