@@ -766,7 +766,7 @@ __device__ __forceinline__ void jitm_task(const BlkTask &t, const double *__rest
     }
 }
 
-// Launch unit u of the norm-form launch: u < nbig runs task u on the whole
+// Launch unit u of a tiled launch: u < nbig runs task u on the whole
 // workgroup, later units run tasks nbig + 4 (u - nbig) + w, one per wave.
 template <int KID>
 __device__ __forceinline__ void blocks_jitm_unit(const BlkTask *__restrict__ tasks, int u, int nbig, int ntask,
