@@ -76,6 +76,43 @@ __device__ __forceinline__ void rpy_coef(double dx, double dy, double dz,
     }
 }
 
+// 1/sqrt(q) for q > 0 in FP64 (defined below).
+__device__ __forceinline__ double rsqrt_refined(double q);
+
+// RPY coefficients as rpy_coef, for the matvec's kernel-evaluating launch:
+// 1/r from the refined reciprocal square root and the radius divisions as
+// products of the points' reciprocal radii (ita = 1/ta, isa = 1/sa), which
+// leaves no FP64 division or square root in the pair loop; the result
+// differs from the divided form by a few ulp.
+__device__ __forceinline__ void rpy_coef_rcp(double dx, double dy, double dz,
+                                             double ta, double sa, double ita, double isa, double C,
+                                             double &t1, double &t2,
+                                             double &ux, double &uy, double &uz) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const bool z = r2 == 0.0;
+    const double inv_r = z ? 0.0 : rsqrt_refined(z ? 1.0 : r2);
+    const double r = r2 * inv_r;
+    const double inv_r2 = inv_r * inv_r;
+    ux = dx * inv_r; uy = dy * inv_r; uz = dz * inv_r;
+    const double tps = ta + sa, tms = ta - sa;
+    if (r > tps) {
+        const double tmp0 = C * 0.75 * inv_r;
+        const double tmp1 = (ta * ta + sa * sa) * inv_r2;
+        t1 = tmp0 * (1.0 + tmp1 * (1.0 / 3.0));
+        t2 = tmp0 * (1.0 - tmp1);
+    } else if (r > fabs(tms)) {
+        const double tmp0 = tms * tms;
+        const double tmp1 = (inv_r2 * inv_r * C) * (ita * isa * (1.0 / 32.0));
+        const double a = tmp0 + 3.0 * r2;
+        t1 = tmp1 * (16.0 * r2 * r * tps - a * a);
+        const double b = tmp0 - r2;
+        t2 = tmp1 * 3.0 * b * b;
+    } else {
+        t1 = C * fmin(ita, isa);
+        t2 = 0.0;
+    }
+}
+
 // 2^x for x <= 0 in FP64: x = n + f with n = rint(x), f in [-1/2, 1/2]
 // (exact, the reduction is in base 2), 2^f by a degree-10 polynomial
 // interpolating 2^f at the 11 Chebyshev nodes of [-1/2, 1/2] (coefficients

@@ -335,114 +335,142 @@ __global__ void k_f64_to_f32(const double *__restrict__ in, float *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Grouped kernel-evaluating bi-matvec (JIT), the krnl_bimv / krnl_eval
-// products of src/H2Pack_matvec.c:586-764 (B) and :1002-1145 (D).  Lanes own
-// source points (NS strips of 64), the four waves take every fourth target
-// point.  KD = 1 for scalar kernels, 3 for RPY.
-template <int KID, int KD, int NS>
-__device__ __forceinline__ void blk_jit_pass(const BlkTask &t, int pbase, int npts,
-                                             const double *__restrict__ cx,
-                                             long long cstride,
-                                             const KParam kp,
-                                             const double *__restrict__ shifts,
-                                             const double *__restrict__ xin,
-                                             double *__restrict__ arena,
-                                             double (*red)[KD][4][H2P_WAVE], bool first) {
+// Grouped kernel-evaluating bi-matvec for RPY (JIT), the krnl_bimv products
+// of src/H2Pack_matvec.c:586-764 (B) and :1002-1145 (D) with the 3x3 tensor
+// K = t1 I + t2 u u^T per point pair.
+//  * Source points live in lanes, NS strips of 64 per pass, with their
+//    coordinates (plus the task's periodic shift), radius and its
+//    reciprocal in registers.
+//  * Target points are staged in LDS per chunk (coordinates, radius, its
+//    reciprocal, the transposed product's input); each wave takes four
+//    targets per iteration and reduces their twelve row sums with three
+//    butterflies (one per component).
+//  * Lanes without a source point use a valid point with zero coefficients.
+#define H2P_JIT_ROWS 256                 // target rows staged per chunk (all JIT launches)
+#define H2P_JIT_MAXNS 4
+#define H2P_RPY_MAXNS 2
+struct RpyRow { double x, y, z, a, ia, v0, v1, v2; };
+
+template <int NS>
+__device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
+                                         const double *__restrict__ cx, long long cstride, const KParam &kp,
+                                         const double *__restrict__ shifts, const double *__restrict__ xin,
+                                         double *__restrict__ arena, RpyRow *rows,
+                                         double (*red)[3][4][H2P_WAVE], bool first) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
     // set 1 may be a periodic image: its coordinates plus the task's shift
     const double hx = shifts[3 * t.shift], hy = shifts[3 * t.shift + 1], hz = shifts[3 * t.shift + 2];
-    double sx[NS], sy[NS], sz[NS], sa[NS];
-    double x0[NS][KD], acc1[NS][KD];
+    double sx[NS], sy[NS], sz[NS], sa[NS], isa[NS], x0[NS][3], acc1[NS][3];
 #pragma unroll
     for (int k = 0; k < NS; k++) {
-        int p = k * H2P_WAVE + lane;
-        bool ok = p < npts;
-        int g = t.c1 + pbase + (ok ? p : 0);
-        sx[k] = cx[g] + hx; sy[k] = cy[g] + hy; sz[k] = cz[g] + hz;
-        if constexpr (KID == KR) sa[k] = ca[g];
+        const int p = k * H2P_WAVE + lane;
+        const bool ok = p < npts;
+        const int g = t.c1 + pbase + (ok ? p : 0);
+        sx[k] = cx[g] + hx;
+        sy[k] = cy[g] + hy;
+        sz[k] = cz[g] + hz;
+        sa[k] = ca[g];
+        isa[k] = 1.0 / sa[k];
 #pragma unroll
-        for (int b = 0; b < KD; b++) {
-            x0[k][b] = ok ? xin[t.in0 + (pbase + p) * KD + b] : 0.0;
+        for (int b = 0; b < 3; b++) {
+            x0[k][b] = ok ? xin[t.in0 + (pbase + p) * 3 + b] : 0.0;
             acc1[k][b] = 0.0;
         }
     }
     const bool has1 = t.flags & 1;
-    for (int i = w; i < t.n0; i += 4) {
-        const int gi = t.c0 + i;
-        const double tx = cx[gi], ty = cy[gi], tz = cz[gi];
-        double v1[KD];
-#pragma unroll
-        for (int a = 0; a < KD; a++) v1[a] = has1 ? xin[t.in1 + i * KD + a] : 0.0;
-        double p[KD];
-#pragma unroll
-        for (int a = 0; a < KD; a++) p[a] = 0.0;
-#pragma unroll
-        for (int k = 0; k < NS; k++) {
-            const bool ok = (k * H2P_WAVE + lane) < npts;
-            double dx = tx - sx[k], dy = ty - sy[k], dz = tz - sz[k];
-            if constexpr (KD == 1) {
-                double kv = kval<KID>(dx * dx + dy * dy + dz * dz, kp);
-                kv = ok ? kv : 0.0;
-                p[0] += kv * x0[k][0];
-                acc1[k][0] += kv * v1[0];
-            } else {
-                const double ta = ca[gi];
-                double t1, t2, ux, uy, uz;
-                rpy_coef(dx, dy, dz, ta, sa[k], kp.p1, t1, t2, ux, uy, uz);
-                t1 = ok ? t1 : 0.0;
-                t2 = ok ? t2 : 0.0;
-                // K = t1 I + t2 u u^T (symmetric)
-                double ud0 = ux * x0[k][0] + uy * x0[k][1] + uz * x0[k][2];
-                double ud1 = ux * v1[0] + uy * v1[1] + uz * v1[2];
-                p[0] += t1 * x0[k][0] + t2 * ux * ud0;
-                p[1] += t1 * x0[k][1] + t2 * uy * ud0;
-                p[2] += t1 * x0[k][2] + t2 * uz * ud0;
-                acc1[k][0] += t1 * v1[0] + t2 * ux * ud1;
-                acc1[k][1] += t1 * v1[1] + t2 * uy * ud1;
-                acc1[k][2] += t1 * v1[2] + t2 * uz * ud1;
-            }
+    const double C = kp.p1;
+    for (int rc = 0; rc < t.n0; rc += H2P_JIT_ROWS) {
+        const int nr = min(t.n0 - rc, H2P_JIT_ROWS);
+        const int nr_pad = (nr + 15) & ~15;
+        __syncthreads();
+        for (int q = threadIdx.x; q < nr_pad; q += blockDim.x) {
+            const bool ok = q < nr;
+            const int g = t.c0 + rc + (ok ? q : 0);
+            RpyRow r;
+            r.x = cx[g];
+            r.y = cy[g];
+            r.z = cz[g];
+            r.a = ca[g];
+            r.ia = 1.0 / r.a;
+            const bool v = ok && has1;
+            r.v0 = v ? xin[t.in1 + (rc + q) * 3] : 0.0;
+            r.v1 = v ? xin[t.in1 + (rc + q) * 3 + 1] : 0.0;
+            r.v2 = v ? xin[t.in1 + (rc + q) * 3 + 2] : 0.0;
+            rows[q] = r;
         }
+        __syncthreads();
+        for (int r0 = 4 * w; r0 < nr_pad; r0 += 16) {
+            double p[4][3];
+            // one target at a time (the barriers bound the registers the
+            // scheduler may spend on interleaving targets)
 #pragma unroll
-        for (int a = 0; a < KD; a++) {
-            double s = wave_sum(p[a]);
-            if (lane == 0) {
-                if (first) arena[t.out0 + i * KD + a] = s;
-                else arena[t.out0 + i * KD + a] += s;
+            for (int a = 0; a < 4; a++) {
+                __builtin_amdgcn_sched_barrier(0);
+                const RpyRow R = rows[r0 + a];
+                p[a][0] = p[a][1] = p[a][2] = 0.0;
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    const bool ok = (k * H2P_WAVE + lane) < npts;
+                    double t1, t2, ux, uy, uz;
+                    rpy_coef_rcp(R.x - sx[k], R.y - sy[k], R.z - sz[k], R.a, sa[k], R.ia, isa[k], C,
+                                 t1, t2, ux, uy, uz);
+                    t1 = ok ? t1 : 0.0;
+                    t2 = ok ? t2 : 0.0;
+                    // K = t1 I + t2 u u^T (symmetric)
+                    const double ud0 = ux * x0[k][0] + uy * x0[k][1] + uz * x0[k][2];
+                    const double ud1 = ux * R.v0 + uy * R.v1 + uz * R.v2;
+                    p[a][0] += t1 * x0[k][0] + t2 * ux * ud0;
+                    p[a][1] += t1 * x0[k][1] + t2 * uy * ud0;
+                    p[a][2] += t1 * x0[k][2] + t2 * uz * ud0;
+                    acc1[k][0] += t1 * R.v0 + t2 * ux * ud1;
+                    acc1[k][1] += t1 * R.v1 + t2 * uy * ud1;
+                    acc1[k][2] += t1 * R.v2 + t2 * uz * ud1;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const int myrow = r0 + ((lane >> 4) & 3);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double s = wave_sum4<double>(p[0][c], p[1][c], p[2][c], p[3][c]);
+                if ((lane & 15) == 0 && myrow < nr) {
+                    double *o = arena + t.out0 + (rc + myrow) * 3 + c;
+                    if (first) *o = s;
+                    else *o += s;
+                }
             }
         }
     }
     if (has1) {
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < NS; k++)
 #pragma unroll
-            for (int b = 0; b < KD; b++) red[k][b][w][lane] = acc1[k][b];
+            for (int b = 0; b < 3; b++) red[k][b][w][lane] = acc1[k][b];
         __syncthreads();
-        for (int q = threadIdx.x; q < npts * KD; q += blockDim.x) {
-            int pnt = q / KD, b = q - pnt * KD;
-            int k = pnt >> 6, l = pnt & 63;
-            arena[t.out1 + (pbase + pnt) * KD + b] =
+        for (int q = threadIdx.x; q < npts * 3; q += blockDim.x) {
+            const int pnt = q / 3, b = q - pnt * 3;
+            const int k = pnt >> 6, l = pnt & 63;
+            arena[t.out1 + (pbase + pnt) * 3 + b] =
                 (red[k][b][0][l] + red[k][b][1][l]) + (red[k][b][2][l] + red[k][b][3][l]);
         }
-        __syncthreads();
     }
 }
 
-#define H2P_JIT_MAXNS 4
 template <int KID, int KD>
 __device__ __forceinline__ void blocks_jit_body(const BlkTask &t, const double *__restrict__ cx,
                                                 long long cstride, const KParam &kp,
                                                 const double *__restrict__ shifts,
                                                 const double *__restrict__ xin,
                                                 double *__restrict__ arena) {
-    __shared__ double red[H2P_JIT_MAXNS][KD][4][H2P_WAVE];
-    for (int pb = 0; pb < t.n1; pb += H2P_JIT_MAXNS * H2P_WAVE) {
-        const int np = min(t.n1 - pb, H2P_JIT_MAXNS * H2P_WAVE);
-        const int ns = (np + 63) >> 6;
+    static_assert(KID == KR && KD == 3, "the per-pass launch is the RPY kernel's");
+    __shared__ RpyRow rows[H2P_JIT_ROWS];
+    __shared__ double red[H2P_RPY_MAXNS][3][4][H2P_WAVE];
+    for (int pb = 0; pb < t.n1; pb += H2P_RPY_MAXNS * H2P_WAVE) {
+        const int np = min(t.n1 - pb, H2P_RPY_MAXNS * H2P_WAVE);
         const bool first = (pb == 0);
-        if (ns <= 1) blk_jit_pass<KID, KD, 1>(t, pb, np, cx, cstride, kp, shifts, xin, arena, red, first);
-        else if (ns <= 2) blk_jit_pass<KID, KD, 2>(t, pb, np, cx, cstride, kp, shifts, xin, arena, red, first);
-        else blk_jit_pass<KID, KD, 4>(t, pb, np, cx, cstride, kp, shifts, xin, arena, red, first);
+        if (np <= H2P_WAVE) rpy_pass<1>(t, pb, np, cx, cstride, kp, shifts, xin, arena, rows, red, first);
+        else rpy_pass<2>(t, pb, np, cx, cstride, kp, shifts, xin, arena, rows, red, first);
         __syncthreads();
     }
 }
@@ -481,7 +509,6 @@ __global__ void __launch_bounds__(256) k_root_block(const double *__restrict__ p
 //    iteration and reduces the four row sums with one butterfly.
 //  * Lanes without a source point use a valid point with x = 0, so every
 //    kernel value stays finite and no per-pair select is needed.
-#define H2P_JIT_ROWS 256                 // target rows staged per chunk
 struct JitRow { double x, y, z, v; };    // coordinates and the transposed product's input
 
 template <int NS>
