@@ -83,7 +83,7 @@ struct h2p_hip_handle {
     long long CT = 0;        // unified coordinate stride (points + skeleton points)
     KParam kp{};
     // device arrays
-    double *d_coord = nullptr;   // xpt_dim x CT
+    double *d_coord = nullptr;   // xpt_dim x CT (RPY: plus a row of reciprocal radii)
     double *d_U = nullptr, *d_UT = nullptr;
     void *d_blob = nullptr;      // AOT B then D (double, or float in FP32-mixed mode)
     double *d_xin = nullptr;     // N + S
@@ -458,7 +458,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if (g->pt_dim != 3) return fail(H2P_HIP_EINVAL, "pt_dim must be 3");
     if ((krnl->id == KR) != (kd == 3) || (kd != 1 && kd != 3))
         return fail(H2P_HIP_EINVAL, "krnl_dim does not match the kernel");
-    if (krnl->id == KR && g->xpt_dim < 4) return fail(H2P_HIP_EINVAL, "RPY needs xpt_dim 4");
+    if (krnl->id == KR && g->xpt_dim != 4) return fail(H2P_HIP_EINVAL, "RPY needs xpt_dim 4 (x, y, z, radius)");
     if (g->n_point <= 0 || g->krnl_mat_size != kd * g->n_point || g->n_node <= 0)
         return fail(H2P_HIP_EINVAL, "inconsistent sizes");
     if ((long long)g->n_point * kd > 0x7fffffffLL) return fail(H2P_HIP_EINVAL, "too many points");
@@ -555,7 +555,10 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     const long long CT = g->n_point + jtot;
     h->CT = CT;
     const int xd = g->xpt_dim;
-    std::vector<double> cu((size_t)xd * CT);
+    // RPY: one more row, the reciprocal radii, so that the kernel-evaluating
+    // launches need no FP64 division per pair
+    const bool rcp_row = krnl->id == KR;
+    std::vector<double> cu((size_t)(xd + (rcp_row ? 1 : 0)) * CT);
     for (int d = 0; d < xd; d++) {
         std::memcpy(cu.data() + (size_t)d * CT, g->coord + (size_t)d * g->n_point, sizeof(double) * g->n_point);
         for (int i = 0; i < nn; i++) {
@@ -564,6 +567,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             std::memcpy(cu.data() + (size_t)d * CT + g->n_point + joff[i], g->J_coord[i] + (size_t)d * m, sizeof(double) * m);
         }
     }
+    if (rcp_row)
+        for (long long i = 0; i < CT; i++) cu[(size_t)4 * CT + i] = 1.0 / cu[(size_t)3 * CT + i];
     // ---- upward sweep tasks per level
     std::vector<FwdTask> fwdt;
     for (int lv = g->max_level; lv >= g->min_adm_level; lv--) {

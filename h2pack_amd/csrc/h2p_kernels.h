@@ -358,7 +358,8 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
                                          double *__restrict__ arena, RpyRow *rows,
                                          double (*red)[3][4][H2P_WAVE], bool first) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+    // rows of cx: x, y, z, radius, reciprocal radius
+    const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride, *cia = cx + 4 * cstride;
     // set 1 may be a periodic image: its coordinates plus the task's shift
     const double hx = shifts[3 * t.shift], hy = shifts[3 * t.shift + 1], hz = shifts[3 * t.shift + 2];
     double sx[NS], sy[NS], sz[NS], sa[NS], isa[NS], x0[NS][3], acc1[NS][3];
@@ -371,7 +372,7 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
         sy[k] = cy[g] + hy;
         sz[k] = cz[g] + hz;
         sa[k] = ca[g];
-        isa[k] = 1.0 / sa[k];
+        isa[k] = cia[g];
 #pragma unroll
         for (int b = 0; b < 3; b++) {
             x0[k][b] = ok ? xin[t.in0 + (pbase + p) * 3 + b] : 0.0;
@@ -392,7 +393,7 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
             r.y = cy[g];
             r.z = cz[g];
             r.a = ca[g];
-            r.ia = 1.0 / r.a;
+            r.ia = cia[g];
             const bool v = ok && has1;
             r.v0 = v ? xin[t.in1 + (rc + q) * 3] : 0.0;
             r.v1 = v ? xin[t.in1 + (rc + q) * 3 + 1] : 0.0;

@@ -96,7 +96,8 @@ struct MMStage {
                                           const double *__restrict__ BDblob, const double *__restrict__ cx,
                                           long long cstride, const KParam &kp,
                                           const double *__restrict__ shifts, const MMPanels &P) {
-        const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride;
+        // rows of cx: x, y, z and, for the 3x3 kernel, radius and reciprocal radius
+        const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride, *cia = cx + 4 * cstride;
         if (s.kind == MM_A_JIT) {
             constexpr int RP = RC / KD, CP = KC / KD;
             const int p0 = mc / KD, q0 = kc / KD;
@@ -113,7 +114,7 @@ struct MMStage {
                     a[u] = ok ? kval<KID>(dx * dx + dy * dy + dz * dz, kp) : 0.0;
                 } else {
                     double t1, t2, ux, uy, uz;
-                    rpy_coef(dx, dy, dz, ca[gi], ca[gj], kp.p1, t1, t2, ux, uy, uz);
+                    rpy_coef_rcp(dx, dy, dz, ca[gi], ca[gj], cia[gi], cia[gj], kp.p1, t1, t2, ux, uy, uz);
                     const double uu[3] = {ux, uy, uz};
 #pragma unroll
                     for (int p = 0; p < 3; p++)
