@@ -590,7 +590,10 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                 if (sum != t.nrow) return bail(fail(H2P_HIP_EINVAL, "non-leaf U_nrow != sum of child ranks"));
                 t.in_off = N + soff[g->children[(size_t)n * mc]];
             }
-            fwdt.push_back(t);
+            for (int c0 = 0; c0 < t.r; c0 += 4 * H2P_WAVE) {
+                t.col0 = c0;
+                fwdt.push_back(t);
+            }
         }
         L.n = (int)fwdt.size() - L.off;
         if (L.n > 0) h->fwd_levels.push_back(L);
@@ -930,7 +933,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         if (XIN > 0x7fffffffLL) return bail(fail(H2P_HIP_EINVAL, "vector space exceeds 2^31 entries"));
         auto in = [](long long off, long long len, long long cap) { return off >= 0 && len >= 0 && off + len <= cap; };
         for (const FwdTask &t : fwdt)
-            if (!in(t.in_off, t.nrow, XIN) || !in(t.out_off, t.r, XIN) || !in(t.u_off, (long long)t.nrow * t.r, utot))
+            if (!in(t.in_off, t.nrow, XIN) || !in(t.out_off, t.r, XIN) || !in(t.u_off, (long long)t.nrow * t.r, utot) ||
+                t.col0 < 0 || t.col0 >= t.r)
                 return bail(fail(H2P_HIP_EINVAL, "upward task out of range"));
         for (const BwdTask2 &t : bwdt)
             if (!in(t.y1_off, t.r, Y1) || !in(t.out_off, t.nrow, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot) ||

@@ -29,6 +29,8 @@ struct BlkTask {
 struct FwdTask {
     long long u_off;
     int nrow, r, in_off, out_off;
+    int col0;         // first of the (at most 4 x 64) output columns this task computes
+    int pad;
 };
 
 // Segmented reduction into y1: y1[dst + e] = sum_k arena[contrib[k] + e].
@@ -63,10 +65,11 @@ __global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ t
 // ---------------------------------------------------------------------------
 // Sweep kernels.
 //
-// Upward sweep (src/H2Pack_matvec.c:95-177): one workgroup per node; lanes own up to 256 columns of U at
-// once (4 strips), the four waves take contiguous quarters of the rows and
-// keep 4 rows x 4 strips of loads in flight; the four wave partials are
-// added in LDS in a fixed order.
+// Upward sweep (src/H2Pack_matvec.c:95-177): one workgroup per node and
+// 256-column chunk of its U (so a node with a large basis spreads over
+// several workgroups); lanes own the chunk's columns (4 strips), the four
+// waves take contiguous quarters of the rows and keep 4 rows x 4 strips of
+// loads in flight; the four wave partials are added in LDS in a fixed order.
 __device__ __forceinline__ void fwd2_body(const FwdTask &t, const double *__restrict__ U,
                                           double *__restrict__ xin) {
     __shared__ double red[4][4][H2P_WAVE];
@@ -75,7 +78,8 @@ __device__ __forceinline__ void fwd2_body(const FwdTask &t, const double *__rest
     const double *in = xin + t.in_off;
     const int q = (t.nrow + 3) >> 2;
     const int i0 = w * q, i1 = min(t.nrow, i0 + q);
-    for (int cs = 0; cs < t.r; cs += 4 * H2P_WAVE) {
+    {
+        const int cs = t.col0;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         int cc[4];
         bool ok[4];
