@@ -254,6 +254,7 @@ __global__ void __launch_bounds__(256) k_cg_init(const int *__restrict__ first_b
 }
 
 __global__ void k_cg_init_finish(int n, const CGCtx *__restrict__ cgs) {
+    if ((int)blockIdx.x >= n) return;   // one workgroup per request
     const CGCtx &c = cgs[blockIdx.x];
     if (threadIdx.x != 0) return;
     double rr = 0.0, bb = 0.0;
@@ -304,6 +305,7 @@ __global__ void __launch_bounds__(256) k_cg_dot_ps(const int *__restrict__ first
 
 // alpha = rr / (p . s), one workgroup per request, partials summed in order
 __global__ void k_cg_alpha(int n, const CGCtx *__restrict__ cgs) {
+    if ((int)blockIdx.x >= n) return;   // one workgroup per request
     const CGCtx &c = cgs[blockIdx.x];
     if (threadIdx.x != 0) return;
     CGScal *sc = c.sc;
@@ -338,6 +340,7 @@ __global__ void __launch_bounds__(256) k_cg_update(const int *__restrict__ first
 
 // rr update, iteration count, stop test (pcg.c:78, 121-124)
 __global__ void k_cg_finish(int n, const CGCtx *__restrict__ cgs) {
+    if ((int)blockIdx.x >= n) return;   // one workgroup per request
     const CGCtx &c = cgs[blockIdx.x];
     if (threadIdx.x != 0) return;
     CGScal *sc = c.sc;
