@@ -17,6 +17,7 @@ SPECS = {
     # taller than one 256-row chunk
     "gaussian_leaf300": (C.KRNL_GAUSSIAN, [1.0], 17000, 10.0, 300, 1e-6, "numerical"),
     "matern32_leaf300": (C.KRNL_MATERN32, [1.0], 17000, 10.0, 300, 1e-6, "numerical"),
+    "rpy_leaf300": (C.KRNL_RPY, [1.0], 17000, 0.03, 300, 1e-7, "surface"),
     "rpy": (C.KRNL_RPY, [1.0], 2500, 0.03, 120, 1e-7, "surface"),
     # tiny trees for dense assembly
     "tiny_coulomb": (C.KRNL_COULOMB, [], 900, 1.0, 40, 1e-8, "surface"),

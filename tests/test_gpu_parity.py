@@ -15,7 +15,8 @@ from tests.problems import problem, rel
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
-KERNELS = ["coulomb", "coulomb_diag", "gaussian", "matern32", "rpy", "gaussian_leaf300", "matern32_leaf300"]
+KERNELS = ["coulomb", "coulomb_diag", "gaussian", "matern32", "rpy", "gaussian_leaf300", "matern32_leaf300",
+           "rpy_leaf300"]
 
 
 def _x(h2, seed=11):
