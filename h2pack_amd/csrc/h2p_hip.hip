@@ -76,6 +76,10 @@ inline bool fwdidx_ok(int v, int n) { return v >= 0 && v < n; }
 
 // profiling events recorded per matvec call (see launch_interior)
 #define H2P_EV_PER_CALL 7
+// share of the D blocks' kernel pairs launched after the B blocks (the
+// smallest blocks), overlapping the skeleton sums and the downward sweep;
+// the rest overlaps the upward sweep.  H2P_HIP_D_LATE overrides it.
+#define H2P_D_LATE 0.5
 
 struct h2p_hip_handle {
     int mode = 0, prec = 0, kid = 0, kd = 1, xpt_dim = 3;
@@ -96,6 +100,7 @@ struct h2p_hip_handle {
     RedTask *d_red = nullptr;
     LeafTask2 *d_leaf = nullptr;
     double *d_yp = nullptr;      // S: push-down values of the downward sweep
+    double *d_yt = nullptr;      // N kd: leaf (U y1) rows of a split handle (k_leaf_u)
     long long *d_contrib = nullptr;
     double *d_mm_x = nullptr, *d_mm_y = nullptr;  // staging of the host-pointer calls
     size_t mm_cap = 0;
@@ -121,13 +126,13 @@ struct h2p_hip_handle {
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
     // grouped block launches: part 0 = B blocks (all blocks when not split),
-    // part 1 = D blocks; each part a contiguous range of d_blk with its
-    // workgroup-unit tasks first
+    // parts 1 and 2 = D blocks launched before and after the B blocks; each
+    // part a contiguous range of d_blk with its workgroup-unit tasks first
     struct BlkPart { int off = 0, n = 0, big = 0, units = 0; };
-    BlkPart part[2];
+    BlkPart part[3];
     bool split = false;             // D blocks on aux_stream, concurrent with the sweeps
     hipStream_t aux_stream = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr;
     long long arena_len = 0;
     size_t device_bytes = 0;
     h2p_hip_info info{};
@@ -161,7 +166,7 @@ static void free_handle(h2p_hip_handle *h) {
     if (!h) return;
     void *ptrs[] = {h->d_coord, h->d_U, h->d_UT, h->d_blob, h->d_xin, h->d_y1, h->d_arena,
                     h->d_fwd, h->d_blk, h->d_fwdt, h->d_bwdt, h->d_red, h->d_leaf,
-                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp, h->d_mmt, h->d_mms, h->d_panels,
+                    h->d_contrib, h->d_mm_x, h->d_mm_y, h->d_yp, h->d_yt, h->d_mmt, h->d_mms, h->d_panels,
                     h->d_cg, h->d_shift, h->d_per};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -175,6 +180,7 @@ static void free_handle(h2p_hip_handle *h) {
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     if (h->aux_stream) (void)hipStreamDestroy(h->aux_stream);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_mid) (void)hipEventDestroy(h->ev_mid);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     delete h;
 }
@@ -198,8 +204,7 @@ static void launch_build(h2p_hip_handle *h, hipStream_t s) {
 // variant 0 (default): the kernel compiled for 4 waves per SIMD; variant 1:
 // compiled without an occupancy target (more registers, 2-3 waves per SIMD)
 template <int KID, typename T, int PART>
-static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s) {
-    const h2p_hip_handle::BlkPart &P = h->part[PART];
+static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s, const h2p_hip_handle::BlkPart &P) {
     if (P.n == 0) return;
     const BlkTask *tasks = h->d_blk + P.off;
     if (h->mv_var == 1)
@@ -212,8 +217,9 @@ static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s) {
 
 template <int KID, typename T>
 static void launch_jit2(h2p_hip_handle *h, hipStream_t s, int part) {
-    if (part == 0) launch_jit2_part<KID, T, 0>(h, s);
-    else launch_jit2_part<KID, T, 1>(h, s);
+    // the D parts share one kernel instantiation (template argument 1)
+    if (part == 0) launch_jit2_part<KID, T, 0>(h, s, h->part[0]);
+    else launch_jit2_part<KID, T, 1>(h, s, h->part[part]);
 }
 
 static void dispatch_jit(h2p_hip_handle *h, hipStream_t s, int part) {
@@ -266,26 +272,39 @@ static void launch_blocks(h2p_hip_handle *h, hipStream_t s, int part) {
 
 // Interior of a matvec: upward sweep, grouped B/D launches, skeleton sums,
 // downward sweep (reads xin[0, N) = px, writes yp / y1 / arena).  On a split
-// handle the D blocks, which read only px, run on aux_stream from the start
-// of the interior and are joined at its end, so their kernel evaluations
-// overlap the HBM-bound sweeps.  With profiling events (ev: 7 per call) every
-// phase runs in order on s instead: ev[1] after the upward sweep, ev[2]
-// after the B blocks, ev[3] after the D blocks, ev[4] after the skeleton
-// sums, ev[5] after the downward sweep.
+// handle the D blocks, which read only px, run on aux_stream and are joined
+// at the end of the interior, so that their kernel evaluations overlap the
+// HBM-bound sweeps: part 1 from the start of the interior (beside the
+// upward sweep), part 2 once the B blocks are done (beside the skeleton
+// sums, the downward sweep and the leaves' U y1 rows, k_leaf_u, which ends
+// the interior of a split handle; k_leaf_d then adds the D contributions).
+// With profiling events (ev: 7 per call) every phase runs in order on s
+// instead: ev[1] after the upward sweep, ev[2] after the B blocks, ev[3]
+// after the D blocks, ev[4] after the skeleton sums, ev[5] after the
+// downward sweep.
 static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
-    const bool fork = h->split && !ev && h->part[1].n > 0;
+    const bool fork = h->split && !ev && (h->part[1].n > 0 || h->part[2].n > 0);
     if (fork) {
         (void)hipEventRecord(h->ev_fork, s);
         (void)hipStreamWaitEvent(h->aux_stream, h->ev_fork, 0);
         launch_blocks(h, h->aux_stream, 1);
-        (void)hipEventRecord(h->ev_join, h->aux_stream);
     }
     for (const Level &L : h->fwd_levels)
         hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
     if (ev) (void)hipEventRecord(ev[1], s);
     launch_blocks(h, s, 0);
     if (ev) (void)hipEventRecord(ev[2], s);
-    if (!fork) launch_blocks(h, s, 1);
+    if (fork) {
+        if (h->part[2].n > 0) {
+            (void)hipEventRecord(h->ev_mid, s);
+            (void)hipStreamWaitEvent(h->aux_stream, h->ev_mid, 0);
+            launch_blocks(h, h->aux_stream, 2);
+        }
+        (void)hipEventRecord(h->ev_join, h->aux_stream);
+    } else {
+        launch_blocks(h, s, 1);
+        launch_blocks(h, s, 2);
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     if (h->n_red > 0)
         hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
@@ -296,6 +315,9 @@ static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
     for (const Level &L : h->bwd_levels)
         hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
     if (ev) (void)hipEventRecord(ev[5], s);
+    if (h->split)
+        hipLaunchKernelGGL(k_leaf_u, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena, h->d_UT,
+                           h->d_yp, h->d_yt);
     if (fork) (void)hipStreamWaitEvent(s, h->ev_join, 0);
 }
 
@@ -334,8 +356,12 @@ static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStre
     } else {
         launch_interior(h, s, ev);
     }
-    hipLaunchKernelGGL(k_leaf_final2, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
-                       h->d_UT, h->d_yp, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
+    if (h->split)
+        hipLaunchKernelGGL(k_leaf_d, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
+                           h->d_yt, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
+    else
+        hipLaunchKernelGGL(k_leaf_final2, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
+                           h->d_UT, h->d_yp, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
     HIP_TRY(hipGetLastError());
     if (prof) {
         HIP_TRY(hipEventRecord(ev[6], s));
@@ -694,12 +720,12 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     h->split = mode == H2P_HIP_JIT && h->kid != KR && !per;
     const bool tiled = mode == H2P_HIP_JIT && h->kid != KR &&
                        (prec == H2P_HIP_FP32MIX || h->kid == KG || h->kid == KM);
-    h->part[0] = h2p_hip_handle::BlkPart{};
-    h->part[1] = h2p_hip_handle::BlkPart{};
+    for (auto &P : h->part) P = h2p_hip_handle::BlkPart{};
     h->part[0].n = h->split ? nB : h->n_blk;
     h->part[1].off = h->part[0].n;
     h->part[1].n = h->n_blk - h->part[0].n;
-    for (auto &P : h->part) {
+    for (int k = 0; k < 2; k++) {
+        auto &P = h->part[k];
         auto b0 = blk.begin() + P.off, b1 = b0 + P.n;
         std::stable_sort(b0, b1, [](const BlkTask &x, const BlkTask &y) {
             return (long long)x.nrow * x.ncol > (long long)y.nrow * y.ncol;
@@ -709,6 +735,28 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             P.big = (int)(std::stable_partition(b0, b1, [](const BlkTask &t) { return !jitm_wave_task(t.n0, t.n1); }) -
                           b0);
         P.units = P.big + (P.n - P.big + 3) / 4;
+    }
+    // the D blocks' last tasks (the smallest, about d_late of their pairs)
+    // form part 2, launched after the B blocks
+    if (h->split && h->part[1].n > 0) {
+        double d_late = H2P_D_LATE;
+        if (const char *e = getenv("H2P_HIP_D_LATE")) d_late = atof(e);
+        const h2p_hip_handle::BlkPart D = h->part[1];
+        double tot = 0.0, acc = 0.0;
+        for (int i = 0; i < D.n; i++) tot += (double)blk[D.off + i].nrow * blk[D.off + i].ncol;
+        int cut = D.n;
+        while (cut > 0 && acc + (double)blk[D.off + cut - 1].nrow * blk[D.off + cut - 1].ncol <= d_late * tot) {
+            cut--;
+            acc += (double)blk[D.off + cut].nrow * blk[D.off + cut].ncol;
+        }
+        auto &P1 = h->part[1], &P2 = h->part[2];
+        P1.n = cut;
+        P1.big = std::min(D.big, cut);
+        P1.units = P1.big + (P1.n - P1.big + 3) / 4;
+        P2.off = D.off + cut;
+        P2.n = D.n - cut;
+        P2.big = std::max(0, D.big - cut);
+        P2.units = P2.big + (P2.n - P2.big + 3) / 4;
     }
     // ---- active nodes of the downward sweep (y1 "set", src/H2Pack_matvec.c:251-256, :802-838)
     std::vector<char> active(nn, 0), has_push(nn, 0);
@@ -1001,6 +1049,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                    (per ? sizeof(double) * per_elems : 0);
         b += sizeof(double) * (std::max<size_t>((size_t)N + S, 1) + 2 * std::max<size_t>(S, 1) +
                                std::max<size_t>((size_t)arena, 1));
+        if (h->split) b += sizeof(double) * std::max<size_t>(N, 1);
         if (mode == H2P_HIP_AOT)
             b += (prec == H2P_HIP_FP32MIX ? sizeof(float) : sizeof(double)) * std::max<size_t>(btot + dtot, 1);
         *dry = b;
@@ -1037,6 +1086,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_yp, (size_t)S, acct))) return bail(rc);
+    if (h->split && (rc = dev_alloc(&h->d_yt, (size_t)N, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_arena, (size_t)arena, acct))) return bail(rc);
     if (hipMemset(h->d_xin, 0, sizeof(double) * ((size_t)N + S)) != hipSuccess)
         return bail(fail(H2P_HIP_EDEVICE, "hipMemset failed"));
@@ -1087,6 +1137,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if (h->split) {
         if (hipStreamCreate(&h->aux_stream) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_mid, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
             return bail(fail(H2P_HIP_EDEVICE, "auxiliary stream / event create failed"));
     }
@@ -1340,13 +1391,10 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
     int err = 0;
     for (int v = 0; v < 2 && !err; v++) {
         h->mv_var = v;
-        launch_blocks(h, s, 0);   // warm-up
-        launch_blocks(h, s, 1);
+        for (int k = 0; k < 3; k++) launch_blocks(h, s, k);   // warm-up
         if (hipEventRecord(e0, s) != hipSuccess) { err = 1; break; }
-        for (int r = 0; r < reps; r++) {
-            launch_blocks(h, s, 0);
-            launch_blocks(h, s, 1);
-        }
+        for (int r = 0; r < reps; r++)
+            for (int k = 0; k < 3; k++) launch_blocks(h, s, k);
         float ms = 0.f;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)

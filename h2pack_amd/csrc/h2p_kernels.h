@@ -176,7 +176,42 @@ struct LeafTask2 {
 };
 // shift != 0 adds shift * x (a diagonal nugget applied outside the H2
 // product, as the PCG wrapper of examples/SPDHSS-H2/pcg_tests.c:15-21 does);
-// px holds x in the permuted order.
+// px holds x in the permuted order.  A leaf's output row i is
+//   ((U y1)_i) + its point-space contributions, in list order, + shift x_i,
+// where y1 = push-down value + skeleton-space contributions, in list order.
+__device__ __forceinline__ void leaf_stage_y1(const LeafTask2 &t, const long long *__restrict__ contrib,
+                                              const double *__restrict__ arena, const double *__restrict__ yp,
+                                              double *yv) {
+    for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
+        double v = t.push ? yp[t.s_off + c] : 0.0;
+        for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
+        yv[c] = v;
+    }
+}
+
+__device__ __forceinline__ double leaf_u_row(const LeafTask2 &t, const double *__restrict__ UT, const double *yv,
+                                             int i) {
+    const double *ut = UT + t.ut_off + i;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = 0;
+    for (; c + 3 < t.r; c += 4) {
+        a0 += ut[(long long)c * t.nrow] * yv[c];
+        a1 += ut[(long long)(c + 1) * t.nrow] * yv[c + 1];
+        a2 += ut[(long long)(c + 2) * t.nrow] * yv[c + 2];
+        a3 += ut[(long long)(c + 3) * t.nrow] * yv[c + 3];
+    }
+    for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
+    return (a0 + a1) + (a2 + a3);
+}
+
+__device__ __forceinline__ double leaf_d_row(const LeafTask2 &t, const long long *__restrict__ contrib,
+                                             const double *__restrict__ arena, const double *__restrict__ px,
+                                             double shift, int i, double v) {
+    for (int k = t.cb0; k < t.ce0; k++) v += arena[contrib[k] + i];
+    if (shift != 0.0) v += shift * px[t.vec_s + i];
+    return v;
+}
+
 __device__ __forceinline__ void leaf_final2_body(const LeafTask2 &t, const long long *__restrict__ contrib,
                                                  const double *__restrict__ arena,
                                                  const double *__restrict__ UT,
@@ -184,28 +219,10 @@ __device__ __forceinline__ void leaf_final2_body(const LeafTask2 &t, const long 
                                                  const int *__restrict__ fwd, double *__restrict__ y,
                                                  const double *__restrict__ px, double shift) {
     __shared__ double yv[H2P_MAXR];
-    for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
-        double v = t.push ? yp[t.s_off + c] : 0.0;
-        for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
-        yv[c] = v;
-    }
+    leaf_stage_y1(t, contrib, arena, yp, yv);
     __syncthreads();
-    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) {
-        const double *ut = UT + t.ut_off + i;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int c = 0;
-        for (; c + 3 < t.r; c += 4) {
-            a0 += ut[(long long)c * t.nrow] * yv[c];
-            a1 += ut[(long long)(c + 1) * t.nrow] * yv[c + 1];
-            a2 += ut[(long long)(c + 2) * t.nrow] * yv[c + 2];
-            a3 += ut[(long long)(c + 3) * t.nrow] * yv[c + 3];
-        }
-        for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
-        double v = (a0 + a1) + (a2 + a3);
-        for (int k = t.cb0; k < t.ce0; k++) v += arena[contrib[k] + i];
-        if (shift != 0.0) v += shift * px[t.vec_s + i];
-        y[fwd[t.vec_s + i]] = v;
-    }
+    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x)
+        y[fwd[t.vec_s + i]] = leaf_d_row(t, contrib, arena, px, shift, i, leaf_u_row(t, UT, yv, i));
 }
 
 __global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict__ tasks,
@@ -217,6 +234,34 @@ __global__ void __launch_bounds__(256) k_leaf_final2(const LeafTask2 *__restrict
                                                      double *__restrict__ y,
                                                      const double *__restrict__ px, double shift) {
     leaf_final2_body(tasks[blockIdx.x], contrib, arena, UT, yp, fwd, y, px, shift);
+}
+
+// The leaf outputs in two launches, for handles whose D blocks are still
+// running during the downward sweep: k_leaf_u writes (U y1)_i to yt (in
+// the permuted order, no D contribution needed), k_leaf_d adds the
+// point-space contributions once the D blocks are done.  Same operations in
+// the same order as k_leaf_final2, so the same bits.
+__global__ void __launch_bounds__(256) k_leaf_u(const LeafTask2 *__restrict__ tasks,
+                                                const long long *__restrict__ contrib,
+                                                const double *__restrict__ arena,
+                                                const double *__restrict__ UT,
+                                                const double *__restrict__ yp, double *__restrict__ yt) {
+    __shared__ double yv[H2P_MAXR];
+    const LeafTask2 t = tasks[blockIdx.x];
+    leaf_stage_y1(t, contrib, arena, yp, yv);
+    __syncthreads();
+    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) yt[t.vec_s + i] = leaf_u_row(t, UT, yv, i);
+}
+
+__global__ void __launch_bounds__(256) k_leaf_d(const LeafTask2 *__restrict__ tasks,
+                                                const long long *__restrict__ contrib,
+                                                const double *__restrict__ arena,
+                                                const double *__restrict__ yt,
+                                                const int *__restrict__ fwd, double *__restrict__ y,
+                                                const double *__restrict__ px, double shift) {
+    const LeafTask2 t = tasks[blockIdx.x];
+    for (int i = threadIdx.x; i < t.nrow; i += blockDim.x)
+        y[fwd[t.vec_s + i]] = leaf_d_row(t, contrib, arena, px, shift, i, yt[t.vec_s + i]);
 }
 
 // ---------------------------------------------------------------------------

@@ -175,15 +175,21 @@ def test_graph_replays_match_eager_path(mode):
         assert np.array_equal(y_graph, y_eager), it
 
 
+@pytest.mark.parametrize("d_late", [None, "0", "1"])
 @pytest.mark.parametrize("name,prec", [("gaussian", "fp64"), ("coulomb", "fp64"), ("matern32", "fp32mix")])
-def test_split_block_launches_agree(name, prec, monkeypatch):
+def test_split_block_launches_agree(name, prec, d_late, monkeypatch):
     """JIT handles launch the B blocks and the D blocks separately, the D
     blocks on a second stream concurrent with the sweeps (a fork/join inside
-    the captured interior, or eager with graphs off).  Both must equal the
-    profiled path, which runs every launch in order on one stream, bit for
-    bit, and the profiled D launch must carry the near-field work."""
+    the captured interior, or eager with graphs off): part of the D blocks
+    from the start, the rest (H2P_HIP_D_LATE of their pairs: the default,
+    none, all) after the B blocks, and the leaf outputs in two launches.
+    Both must equal the profiled path, which runs every launch in order on
+    one stream, bit for bit, and the profiled D launch must carry the
+    near-field work."""
     import torch
     h2, _ = problem(name)
+    if d_late is not None:
+        monkeypatch.setenv("H2P_HIP_D_LATE", d_late)
     g = H2MatHIP(h2, mode="jit", prec=prec)
     monkeypatch.setenv("H2P_HIP_NO_GRAPH", "1")
     e = H2MatHIP(h2, mode="jit", prec=prec)
