@@ -872,9 +872,13 @@ __device__ __forceinline__ void blocks_jitm_unit(const BlkTask *__restrict__ tas
 // folded into FP64 registers.
 struct JitfPt { float x, y, z, v; };   // prepared coordinates and input value
 
+// A workgroup unit's pass covers up to H2P_JITF_NT tiles of 16 columns
+// (one source per thread), a wave unit's up to H2P_JITM_NT: the wider pass
+// halves the per-column-tile cost of the row reductions and row staging.
 // LDS in doubles: workgroup unit rows 2 H2P_JIT_ROWS, sources 2 x 16 NT,
-// wave partials 4 x NT x 64; a wave unit 2 H2P_JITW_ROWS + 2 x 16 NT per wave.
-#define H2P_JITF_LDS_WG (2 * H2P_JIT_ROWS + 2 * 16 * H2P_JITM_NT + 4 * H2P_JITM_NT * H2P_WAVE)
+// wave partials 4 x NT x 16; a wave unit 2 H2P_JITW_ROWS + 2 x 16 NT per wave.
+#define H2P_JITF_NT 16
+#define H2P_JITF_LDS_WG (2 * H2P_JIT_ROWS + 2 * 16 * H2P_JITF_NT + 4 * H2P_JITF_NT * 16)
 #define H2P_JITF_LDS_WAVE (2 * H2P_JITW_ROWS + 2 * 16 * H2P_JITM_NT)
 #define H2P_JITF_LDS (H2P_JITF_LDS_WG > 4 * H2P_JITF_LDS_WAVE ? H2P_JITF_LDS_WG : 4 * H2P_JITF_LDS_WAVE)
 
@@ -901,12 +905,8 @@ __device__ __forceinline__ void jitf_pass(const BlkTask &t, int pbase, int npts,
         srcs[q] = e;
     }
     float acc[NT];
-    double accd[NT];
 #pragma unroll
-    for (int c = 0; c < NT; c++) {
-        acc[c] = 0.0f;
-        accd[c] = 0.0;
-    }
+    for (int c = 0; c < NT; c++) acc[c] = 0.0f;
     const bool has1 = t.flags & 1;
     for (int rc = 0; rc < t.n0; rc += ROWS) {
         const int nr = min(t.n0 - rc, ROWS);
@@ -957,33 +957,31 @@ __device__ __forceinline__ void jitf_pass(const BlkTask &t, int pbase, int npts,
                 else *o += (double)s;
             }
         }
+        if (has1) {
+            // the chunk's FP32 column sums in FP64, (g0 + g1) + (g2 + g3)
+            // over the four lane groups, added in chunk order to the wave's
+            // sums (in LDS for a workgroup unit, in the output for a wave unit)
 #pragma unroll
-        for (int c = 0; c < NT; c++) {   // fold the chunk's FP32 column sums
-            accd[c] += (double)acc[c];
-            acc[c] = 0.0f;
+            for (int c = 0; c < NT; c++) {
+                double v = (double)acc[c];
+                v += __shfl_xor(v, 16, H2P_WAVE);
+                v += __shfl_xor(v, 32, H2P_WAVE);
+                acc[c] = 0.0f;
+                const int j = c * 16 + li;
+                double *o = NW == 4 ? red + (w * NT + c) * 16 + li : arena + t.out1 + pbase + j;
+                if (lg == 0 && (NW == 4 || j < npts)) *o = rc == 0 ? v : *o + v;
+            }
         }
     }
     if (!has1) return;
-    if constexpr (NW == 4) {
-#pragma unroll
-        for (int c = 0; c < NT; c++) red[(w * NT + c) * H2P_WAVE + lane] = accd[c];
+    if constexpr (NW == 4) {   // the four waves in order
         __syncthreads();
         for (int j = threadIdx.x; j < npts; j += blockDim.x) {
             const int c = j >> 4, m = j & 15;
-            double s = 0.0;
+            double s = red[c * 16 + m];
 #pragma unroll
-            for (int w2 = 0; w2 < 4; w2++)
-#pragma unroll
-                for (int g = 0; g < 4; g++) s += red[(w2 * NT + c) * H2P_WAVE + g * 16 + m];
+            for (int w2 = 1; w2 < 4; w2++) s += red[(w2 * NT + c) * 16 + m];
             arena[t.out1 + pbase + j] = s;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < NT; c++) {
-            double s = accd[c] + __shfl_xor(accd[c], 16, H2P_WAVE);
-            s += __shfl_xor(s, 32, H2P_WAVE);
-            const int j = c * 16 + li;
-            if (lg == 0 && j < npts) arena[t.out1 + pbase + j] = s;
         }
     }
 }
@@ -994,21 +992,33 @@ __device__ __forceinline__ void jitf_task(const BlkTask &t, const double *__rest
                                           const double *__restrict__ xin, double *__restrict__ arena,
                                           double *lds) {
     constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
+    constexpr int NTX = NW == 4 ? H2P_JITF_NT : H2P_JITM_NT;
     JitfPt *rows = (JitfPt *)lds;
     JitfPt *srcs = (JitfPt *)(lds + 2 * ROWS);
-    double *red = lds + 2 * ROWS + 2 * 16 * H2P_JITM_NT;   // workgroup unit only
+    double *red = lds + 2 * ROWS + 2 * 16 * NTX;   // workgroup unit only
     const double ox = cx[t.c0], oy = cx[cstride + t.c0], oz = cx[2 * cstride + t.c0];
-    for (int pb = 0; pb < t.n1; pb += H2P_JITM_NT * 16) {
-        const int np = min(t.n1 - pb, H2P_JITM_NT * 16);
+    for (int pb = 0; pb < t.n1; pb += NTX * 16) {
+        const int np = min(t.n1 - pb, NTX * 16);
         const bool first = (pb == 0);
-        switch ((np + 15) >> 4) {
+        const int nt = (np + 15) >> 4;
 #define H2P_JITF_CASE(n) \
             case n: jitf_pass<KID, n, NW>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rows, srcs, red, first); break;
-            H2P_JITF_CASE(1) H2P_JITF_CASE(2) H2P_JITF_CASE(3) H2P_JITF_CASE(4)
-            H2P_JITF_CASE(5) H2P_JITF_CASE(6) H2P_JITF_CASE(7) H2P_JITF_CASE(8)
-#undef H2P_JITF_CASE
-            default: break;
+        if constexpr (NW == 4) {
+            switch (nt) {
+                H2P_JITF_CASE(1) H2P_JITF_CASE(2) H2P_JITF_CASE(3) H2P_JITF_CASE(4)
+                H2P_JITF_CASE(5) H2P_JITF_CASE(6) H2P_JITF_CASE(7) H2P_JITF_CASE(8)
+                H2P_JITF_CASE(9) H2P_JITF_CASE(10) H2P_JITF_CASE(11) H2P_JITF_CASE(12)
+                H2P_JITF_CASE(13) H2P_JITF_CASE(14) H2P_JITF_CASE(15) H2P_JITF_CASE(16)
+                default: break;
+            }
+        } else {
+            switch (nt) {
+                H2P_JITF_CASE(1) H2P_JITF_CASE(2) H2P_JITF_CASE(3) H2P_JITF_CASE(4)
+                H2P_JITF_CASE(5) H2P_JITF_CASE(6) H2P_JITF_CASE(7) H2P_JITF_CASE(8)
+                default: break;
+            }
         }
+#undef H2P_JITF_CASE
         jitm_sync<NW>();
     }
 }
