@@ -8,6 +8,8 @@
 // contiguous range in child order; the stacked operand of a non-leaf U is
 // therefore one contiguous slice of y0 / y1.
 #pragma once
+#include <type_traits>
+
 #include "h2p_device.h"
 
 namespace h2p {
@@ -680,6 +682,18 @@ __device__ __forceinline__ void blocks_jitc(const BlkTask &t, const double *__re
 // n1 <= 16 H2P_JITM_NT), one per wave, with wave-private LDS and no
 // workgroup barriers, so the short blocks do not wait on each other.
 #define H2P_JITM_NT 8
+
+// Runs a tile body for the rows that hold points: lane group g's register r
+// holds tile row 4 r + g, so with vr valid rows the registers r >= ceil(vr /
+// 4) hold only padding (a repeated point with input 0, whose products add
+// exact zeros) and are skipped.  Same results, fewer kernel evaluations in a
+// block's last row tile.
+template <typename F>
+__device__ __forceinline__ void tile_rows(int vr, F &&body) {
+    if (vr <= 4) body(std::integral_constant<int, 1>{});
+    else if (vr <= 8) body(std::integral_constant<int, 2>{});
+    else body(std::integral_constant<int, 4>{});
+}
 #define H2P_JITW_ROWS 80
 struct JitmRow { double a[4]; };   // (1, x, y, z) of a target; (|s|^2, -2 s) of a source
 struct JitmNV { double n, v; };    // |r|^2 and the transposed product's input
@@ -768,19 +782,23 @@ __device__ __forceinline__ void jitm_pass(const BlkTask &t, int pbase, int npts,
             // the scheduling barriers keep the compiler from hoisting all NT
             // products ahead of the VALU work (each live result costs 8
             // registers); other waves cover the product's latency
+            auto tile = [&](auto nrq) {
+                constexpr int NR = decltype(nrq)::value;
 #pragma unroll
-            for (int c = 0; c < NT; c++) {
-                const double xc = srcx[c * 16 + li];
-                const d4 q = __builtin_amdgcn_mfma_f64_16x16x4f64(a, srcb[c * 16 + li].a[lg], cn, 0, 0, 0);
-                __builtin_amdgcn_sched_barrier(0);
+                for (int c = 0; c < NT; c++) {
+                    const double xc = srcx[c * 16 + li];
+                    const d4 q = __builtin_amdgcn_mfma_f64_16x16x4f64(a, srcb[c * 16 + li].a[lg], cn, 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const double kv = kprep<KID>(q[r], diag);
-                    p[r] += kv * xc;
-                    acc[c] += kv * v[r];
+                    for (int r = 0; r < NR; r++) {
+                        const double kv = kprep<KID>(q[r], diag);
+                        p[r] += kv * xc;
+                        acc[c] += kv * v[r];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            };
+            tile_rows(nr - tt * 16, tile);
             const double s = group16_sum4(p[0], p[1], p[2], p[3]);
             const int myrow = tt * 16 + 4 * ((lane >> 2) & 3) + lg;
             if ((lane & 3) == 0 && myrow < nr) {
@@ -936,19 +954,23 @@ __device__ __forceinline__ void jitf_pass(const BlkTask &t, int pbase, int npts,
             }
             // one column tile at a time (the barriers bound the registers
             // the scheduler may spend on interleaving tiles)
+            auto tile = [&](auto nrq) {
+                constexpr int NR = decltype(nrq)::value;
 #pragma unroll
-            for (int c = 0; c < NT; c++) {
-                const JitfPt e = srcs[c * 16 + li];
-                __builtin_amdgcn_sched_barrier(0);
+                for (int c = 0; c < NT; c++) {
+                    const JitfPt e = srcs[c * 16 + li];
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float dx = rx[r] - e.x, dy = ry[r] - e.y, dz = rz[r] - e.z;
-                    const float kv = kprep<KID>(dx * dx + dy * dy + dz * dz, dg);
-                    p[r] += kv * e.v;
-                    acc[c] += kv * rv[r];
+                    for (int r = 0; r < NR; r++) {
+                        const float dx = rx[r] - e.x, dy = ry[r] - e.y, dz = rz[r] - e.z;
+                        const float kv = kprep<KID>(dx * dx + dy * dy + dz * dz, dg);
+                        p[r] += kv * e.v;
+                        acc[c] += kv * rv[r];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            };
+            tile_rows(nr - tt * 16, tile);
             const float s = group16_sum4(p[0], p[1], p[2], p[3]);
             const int myrow = tt * 16 + 4 * ((lane >> 2) & 3) + lg;
             if ((lane & 3) == 0 && myrow < nr) {
