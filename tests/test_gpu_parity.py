@@ -1,6 +1,6 @@
 """GPU parity: libh2p_hip.so against the oracle on the same H2 representation.
 
-Bar (north-star): FP64 results agree with the oracle's restatement of
+Bar: FP64 results agree with the oracle's restatement of
 H2P_matvec to 1e-12 relative 2-norm; FP32-mixed results to the build
 tolerance.  Every call goes through the C ABI (include/h2p_hip.h).
 """
