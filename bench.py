@@ -289,7 +289,9 @@ def main():
                      "traffic_source": "profiles/gaussian_1M_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "note": "achieved = H2Pack JIT_flops model (14 flops per Gaussian pair) / kernel time; "
                              "FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak (one FP64 pipe: "
-                             "SQ_VALU_MFMA_COEXEC_CYCLES = 0)"},
+                             "SQ_VALU_MFMA_COEXEC_CYCLES = 0); evaluating a pair issues ~20 pipe slots "
+                             "(2^x alone 14 FP64 operations), and rocprofv3 counters put the pipe ~90% busy "
+                             "(profiles/gaussian_1M_jit_pipe_counters.md)"},
         "cpu_baseline": cpu,
         "rel_err_vs_direct_nbody_1000rows": err_direct,
         "rel_err_vs_oracle": err_oracle,
