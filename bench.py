@@ -125,7 +125,8 @@ def main():
         r = m.autotune(0)
         r["tune_s"] = time.perf_counter() - t
         log(f"[bench] {label}: autotuned block launch -> variant {r['variants'][0]} "
-            f"(default {r['default_ms']:.3f} ms, chosen {r['tuned_ms']:.3f} ms, {r['tune_s']:.1f}s)")
+            f"(default {r['default_ms']:.3f} ms, chosen {r['tuned_ms']:.3f} ms, {r['tune_s']:.1f}s); "
+            f"D blocks {'beside the sweeps' if r['d_concurrent'] else 'in order'}")
         return r
 
     def timed(m, label):
@@ -184,7 +185,8 @@ def main():
             "speedup_vs_fp64": el_j / el_f,
             "rel_err_vs_fp64_gpu": err_f_fp64,
             "rel_err_vs_direct_nbody_1000rows": err_f_direct,
-            "kernel_ms": info_f["t_blocks"], "block_variant": tune_f["variants"][0]}
+            "kernel_ms": info_f["t_blocks"], "block_variant": tune_f["variants"][0],
+            "d_concurrent": tune_f["d_concurrent"]}
     log(f"[bench] fp32mix-jit: {K / el_f:.2f} matvec/s ({el_j / el_f:.2f}x FP64); rel err vs FP64 GPU "
         f"{err_f_fp64:.3e}, vs direct n-body {err_f_direct:.3e}")
 
@@ -282,6 +284,7 @@ def main():
                      "kernel": ("k_blocks_jit2_occ<1, double, 4, 0>" if jv == 0
                                 else "k_blocks_jit2<1, double, 0>"), "kernel_ms": info_j["t_blocks_b"],
                      "flops_per_launch": flops_b, "block_variant": jv,
+                     "d_concurrent": tune_j["d_concurrent"],
                      "d_launch": {"kernel": ("k_blocks_jit2_occ<1, double, 4, 1>" if jv == 0
                                              else "k_blocks_jit2<1, double, 1>"),
                                   "kernel_ms": info_j["t_blocks_d"], "flops_per_launch": flops_d,

@@ -131,6 +131,7 @@ struct h2p_hip_handle {
     struct BlkPart { int off = 0, n = 0, big = 0, units = 0; };
     BlkPart part[3];
     bool split = false;             // D blocks on aux_stream, concurrent with the sweeps
+    bool d_serial = false;          // split handle whose D blocks run in order (h2p_hip_autotune)
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr;
     long long arena_len = 0;
@@ -283,7 +284,7 @@ static void launch_blocks(h2p_hip_handle *h, hipStream_t s, int part) {
 // after the D blocks, ev[4] after the skeleton sums, ev[5] after the
 // downward sweep.
 static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
-    const bool fork = h->split && !ev && (h->part[1].n > 0 || h->part[2].n > 0);
+    const bool fork = h->split && !h->d_serial && !ev && (h->part[1].n > 0 || h->part[2].n > 0);
     if (fork) {
         (void)hipEventRecord(h->ev_fork, s);
         (void)hipStreamWaitEvent(h->aux_stream, h->ev_fork, 0);
@@ -1357,8 +1358,42 @@ int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, in
     return rc;
 }
 
+// Mean time of the matvec interior (graph replay, or eager launches with
+// graphs off) over reps calls after one warm-up, on the handle's stream.
+static int time_interior(h2p_hip_handle *h, int reps, double *ms) {
+    hipStream_t s = h->own_stream;
+    auto once = [&]() -> int {
+        if (h->use_graph) {
+            int rc = ensure_graph(h);
+            if (rc) return rc;
+            HIP_TRY(hipGraphLaunch(h->graph_exec, s));
+        } else {
+            launch_interior(h, s, nullptr);
+        }
+        return 0;
+    };
+    int rc = once();
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float t = 0.f;
+    rc = hipEventRecord(e0, s) == hipSuccess ? 0 : fail(H2P_HIP_EDEVICE, "hipEventRecord failed");
+    for (int r = 0; r < reps && !rc; r++) rc = once();
+    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        rc = fail(H2P_HIP_EDEVICE, "timing the matvec interior failed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+}
+
 // n_vec == 0: the matvec's grouped block launch (default or occupancy
-// variant; the RPY JIT kernel has one variant).
+// variant; the RPY JIT kernel has one variant), then, on a split handle,
+// whether its D blocks run beside the sweeps on the second stream or in
+// order (concurrency pays when the D blocks cover the sweeps, and can cost
+// time where the sweeps are long and the D blocks few).  Same results.
 static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tuned_ms) {
     if (t_default_ms) *t_default_ms = 0.0;
     if (t_tuned_ms) *t_tuned_ms = 0.0;
@@ -1377,8 +1412,11 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(sig);
         if (it != g_tune_cache.end()) {
-            if (h->mv_var != it->second) drop_matvec_graph(h);
-            h->mv_var = it->second;
+            const int v = it->second & 1;
+            const bool ser = h->split && (it->second & 2);
+            if (h->mv_var != v || h->d_serial != ser) drop_matvec_graph(h);
+            h->mv_var = v;
+            h->d_serial = ser;
             return 0;
         }
     }
@@ -1410,8 +1448,23 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
     h->mv_var = best;
     if (t_default_ms) *t_default_ms = t[0];
     if (t_tuned_ms) *t_tuned_ms = t[best];
+    int code = best;
+    if (h->split) {
+        double ti[2] = {0.0, 0.0};   // D blocks concurrent, in order
+        for (int k = 0; k < 2; k++) {
+            const bool ser = k == 1;
+            if (h->d_serial != ser) drop_matvec_graph(h);
+            h->d_serial = ser;
+            int rc = time_interior(h, 3, &ti[k]);
+            if (rc) return rc;
+        }
+        const bool ser = ti[1] < ti[0];
+        if (h->d_serial != ser) drop_matvec_graph(h);
+        h->d_serial = ser;
+        code |= ser ? 2 : 0;
+    }
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[sig] = best;
+    g_tune_cache[sig] = code;
     return 0;
 }
 
@@ -1541,6 +1594,7 @@ int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info) {
         h->ev_used = 0;
     }
     h->info.mv_variant = h->mv_var;
+    h->info.mv_d_concurrent = (h->split && !h->d_serial) ? 1 : 0;
     *info = h->info;
     return 0;
 }

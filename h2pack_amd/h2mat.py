@@ -164,12 +164,15 @@ class H2MatHIP:
     def autotune(self, n_vec: int) -> dict:
         """Time the kernel variants on this matrix and keep the fastest
         (h2p_hip_autotune): the matmul tiles per phase for ``n_vec`` columns,
-        or with ``n_vec`` = 0 the matvec's grouped block launch."""
+        or with ``n_vec`` = 0 the matvec's grouped block launch and whether
+        its D blocks run beside the sweeps (``d_concurrent``)."""
         t0, t1 = ct.c_double(0), ct.c_double(0)
         _check(self.lib.h2p_hip_autotune(self.handle, int(n_vec), ct.byref(t0), ct.byref(t1)))
         out = {"default_ms": t0.value, "tuned_ms": t1.value}
         if n_vec == 0:
-            out["variants"] = [self.info()["mv_variant"]]
+            inf = self.info()
+            out["variants"] = [inf["mv_variant"]]
+            out["d_concurrent"] = inf["mv_d_concurrent"]
         else:
             out["variants"] = [self.mm_variant(ph, n_vec) for ph in range(4)]
         return out

@@ -127,6 +127,10 @@ typedef struct {
      * kernels (B blocks, D blocks; profiled calls run them in order);
      * elsewhere t_blocks_b = t_blocks and t_blocks_d = 0 */
     double t_blocks_b, t_blocks_d;
+    /* 1 when the D blocks of a JIT handle of the scalar kernels run on a
+     * second stream beside the sweeps, 0 when they run in order
+     * (h2p_hip_autotune) */
+    int mv_d_concurrent;
 } h2p_hip_info;
 
 /* Mirror an H2 representation into HBM.  mode = H2P_HIP_AOT / _JIT,
@@ -169,7 +173,9 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
  * builds of the same body compiled for different occupancies (JIT: 4 waves
  * per SIMD by default, unconstrained registers as the alternative; AOT:
  * unconstrained by default, 5 (FP64) or 8 (FP32) waves as the
- * alternative), again with identical results.
+ * alternative), and on JIT handles of the scalar kernels the D blocks run
+ * either beside the sweeps on a second stream (default) or in order;
+ * again with identical results.
  * (H2Pack has no counterpart: its products are fixed BLAS / SIMD loops.) */
 #define H2P_HIP_MM_VARIANTS 4
 int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms,

@@ -50,8 +50,10 @@ def test_tuning_is_shared_by_identical_histograms():
 @pytest.mark.parametrize("name,mode,prec", [("gaussian", "jit", "fp64"), ("gaussian", "jit", "fp32mix"),
                                             ("coulomb", "aot", "fp64"), ("coulomb", "aot", "fp32mix")])
 def test_matvec_variants_give_the_same_result(name, mode, prec):
-    """The matvec's block launch has an occupancy variant; both variants and
-    the tuned choice give bitwise the same y (graph replays included)."""
+    """The matvec's block launch has an occupancy variant, and JIT handles of
+    the scalar kernels run their D blocks beside the sweeps or in order;
+    every choice gives bitwise the same y (graph replays included), and a
+    second handle of the same block shapes takes the cached choice."""
     h2, _ = problem(name)
     m = H2MatHIP(h2, mode=mode, prec=prec, host_blocks=False)
     x = np.random.default_rng(5).random(h2.krnl_mat_size) - 0.5
@@ -63,5 +65,11 @@ def test_matvec_variants_give_the_same_result(name, mode, prec):
     t = m.autotune(0)
     print(f"[{name} {mode} {prec}] matvec tune {t}", flush=True)
     assert t["variants"][0] in (0, 1)
+    assert t["d_concurrent"] in ((0, 1) if mode == "jit" else (0,))
     assert np.array_equal(m.matvec(x), y0)
     assert np.array_equal(m.matvec(x), y0)
+    m2 = H2MatHIP(h2, mode=mode, prec=prec, host_blocks=False)
+    t2 = m2.autotune(0)
+    assert (t2["variants"][0], t2["d_concurrent"]) == (t["variants"][0], t["d_concurrent"])
+    assert np.array_equal(m2.matvec(x), y0)
+    m2.close()
