@@ -125,8 +125,9 @@ def main():
         r = m.autotune(0)
         r["tune_s"] = time.perf_counter() - t
         log(f"[bench] {label}: autotuned block launch -> variant {r['variants'][0]} "
-            f"(default {r['default_ms']:.3f} ms, chosen {r['tuned_ms']:.3f} ms, {r['tune_s']:.1f}s); "
-            f"D blocks {'beside the sweeps' if r['d_concurrent'] else 'in order'}")
+            f"(default {r['default_ms']:.3f} ms, chosen {r['tuned_ms']:.3f} ms, {r['tune_s']:.1f}s)"
+            + ("" if "aot" in label else
+               f"; D blocks {'beside the sweeps' if r['d_concurrent'] else 'in order'}"))
         return r
 
     def timed(m, label):
