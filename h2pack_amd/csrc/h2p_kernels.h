@@ -181,13 +181,34 @@ struct LeafTask2 {
 // px holds x in the permuted order.  A leaf's output row i is
 //   ((U y1)_i) + its point-space contributions, in list order, + shift x_i,
 // where y1 = push-down value + skeleton-space contributions, in list order.
+// With a small rank (r <= blockDim / 2) the contributions are dealt to
+// G = blockDim / r groups of r threads (group g sums contributions g, g + G,
+// ...), so that G loads are in flight per skeleton entry instead of one
+// dependent chain, and the G group sums are added in group order; part
+// holds blockDim doubles.  The caller synchronises before reading yv.
 __device__ __forceinline__ void leaf_stage_y1(const LeafTask2 &t, const long long *__restrict__ contrib,
                                               const double *__restrict__ arena, const double *__restrict__ yp,
-                                              double *yv) {
-    for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
-        double v = t.push ? yp[t.s_off + c] : 0.0;
-        for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
-        yv[c] = v;
+                                              double *yv, double *part) {
+    const int G = t.r > 0 ? min(8, (int)blockDim.x / t.r) : 1;
+    if (G <= 1) {
+        for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
+            double v = t.push ? yp[t.s_off + c] : 0.0;
+            for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
+            yv[c] = v;
+        }
+        return;
+    }
+    const int g = threadIdx.x / t.r, c = threadIdx.x - g * t.r;
+    if (g < G) {
+        double v = 0.0;
+        for (int k = t.cb1 + g; k < t.ce1; k += G) v += arena[contrib[k] + c];
+        part[g * t.r + c] = v;
+    }
+    __syncthreads();
+    for (int c2 = threadIdx.x; c2 < t.r; c2 += blockDim.x) {
+        double v = t.push ? yp[t.s_off + c2] : 0.0;
+        for (int g2 = 0; g2 < G; g2++) v += part[g2 * t.r + c2];
+        yv[c2] = v;
     }
 }
 
@@ -220,8 +241,8 @@ __device__ __forceinline__ void leaf_final2_body(const LeafTask2 &t, const long 
                                                  const double *__restrict__ yp,
                                                  const int *__restrict__ fwd, double *__restrict__ y,
                                                  const double *__restrict__ px, double shift) {
-    __shared__ double yv[H2P_MAXR];
-    leaf_stage_y1(t, contrib, arena, yp, yv);
+    __shared__ double yv[H2P_MAXR], part[256];
+    leaf_stage_y1(t, contrib, arena, yp, yv, part);
     __syncthreads();
     for (int i = threadIdx.x; i < t.nrow; i += blockDim.x)
         y[fwd[t.vec_s + i]] = leaf_d_row(t, contrib, arena, px, shift, i, leaf_u_row(t, UT, yv, i));
@@ -248,9 +269,9 @@ __global__ void __launch_bounds__(256) k_leaf_u(const LeafTask2 *__restrict__ ta
                                                 const double *__restrict__ arena,
                                                 const double *__restrict__ UT,
                                                 const double *__restrict__ yp, double *__restrict__ yt) {
-    __shared__ double yv[H2P_MAXR];
+    __shared__ double yv[H2P_MAXR], part[256];
     const LeafTask2 t = tasks[blockIdx.x];
-    leaf_stage_y1(t, contrib, arena, yp, yv);
+    leaf_stage_y1(t, contrib, arena, yp, yv, part);
     __syncthreads();
     for (int i = threadIdx.x; i < t.nrow; i += blockDim.x) yt[t.vec_s + i] = leaf_u_row(t, UT, yv, i);
 }
