@@ -8,13 +8,25 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define H2P_HD __device__ __attribute__((always_inline))
+#include "h2p_exp2.h"
+
 #define H2P_WAVE 64
 
 namespace h2p {
 
 typedef double d4 __attribute__((ext_vector_type(4)));   // one lane's share of a 16x16 FP64 MFMA tile
 
-enum { KC = 0, KG = 1, KM = 2, KR = 3 };
+// Kernel ids.  KGW / KMW are Gaussian / Matern-3/2 for FP64 JIT handles
+// whose point extent can push the exponential's argument past the range of
+// the unclamped exp2_neg: their kernel values clamp the argument first
+// (bitwise the same values wherever the unclamped form is in range).
+enum { KC = 0, KG = 1, KM = 2, KR = 3, KGW = 4, KMW = 5 };
+
+__host__ __device__ constexpr int kbase(int kid) { return kid == KGW ? KG : kid == KMW ? KM : kid; }
+
+// the table of exp2_neg, copied into LDS by the kernels that use it
+__constant__ double c_e2tab[H2P_E2_N] = {H2P_E2_TABLE};
 
 struct KParam {
     double p0;      // Coulomb diagonal value / Gaussian l / Matern l / RPY eta
@@ -113,29 +125,6 @@ __device__ __forceinline__ void rpy_coef_rcp(double dx, double dy, double dz,
     }
 }
 
-// 2^x for x <= 0 in FP64: x = n + f with n = rint(x), f in [-1/2, 1/2]
-// (exact, the reduction is in base 2), 2^f by a degree-10 polynomial
-// interpolating 2^f at the 11 Chebyshev nodes of [-1/2, 1/2] (coefficients
-// computed in extended precision; max relative error 4.5e-16 with Horner in
-// FP64), then scaled by 2^n with ldexp, which returns 0 once n drops below
-// the subnormal range.
-__device__ __forceinline__ double exp2_nonpos(double x) {
-    const double n = __builtin_rint(x);
-    const double f = x - n;
-    double p = 7.072585489423359e-09;
-    p = __builtin_fma(p, f, 1.0208690645080226e-07);
-    p = __builtin_fma(p, f, 1.3215442588981224e-06);
-    p = __builtin_fma(p, f, 1.525265725816688e-05);
-    p = __builtin_fma(p, f, 0.00015403530441732765);
-    p = __builtin_fma(p, f, 0.0013333558230168993);
-    p = __builtin_fma(p, f, 0.009618129107606902);
-    p = __builtin_fma(p, f, 0.05550410866444769);
-    p = __builtin_fma(p, f, 0.24022650695910097);
-    p = __builtin_fma(p, f, 0.69314718055995);
-    p = __builtin_fma(p, f, 1.0);
-    return __builtin_amdgcn_ldexp(p, (int)n);
-}
-
 // 1/sqrt(q) for q > 0 in FP64: the hardware estimate refined by one
 // Newton-Raphson step with its second-order term, e = 1 - q y^2,
 // y <- y + y e (1/2 + 3/8 e); the error of the estimate is cubed.
@@ -148,22 +137,28 @@ __device__ __forceinline__ double rsqrt_refined(double q) {
 // Scalar kernels on prepared squared distances.  Coordinates are scaled
 // before the distance is formed so that the kernels need no parameter:
 //   Gaussian exp(-l r^2)          : scale sqrt(l log2 e)   -> K = 2^(-q)
-//   Matern32 (1+t) e^{-t}, t=sqrt3 l r : scale sqrt3 l     -> K = (1+s) 2^(-s log2 e), s = sqrt(q)
+//   Matern32 (1+t) e^{-t}, t=sqrt3 l r : scale sqrt3 l     -> K = (1+s) e^(-s), s = sqrt(q)
 //   Coulomb 1/r                   : scale 1                -> K = q == 0 ? diag : rsqrt(q)
+// e2t is the exp2_neg table (LDS), unused by Coulomb.
 template <int KID>
-__device__ __forceinline__ double kprep(double q, double diag) {
+__device__ __forceinline__ double kprep(double q, double diag, const double *e2t = nullptr) {
     if constexpr (KID == KC) {
         const bool z = q == 0.0;
         const double y = rsqrt_refined(z ? 1.0 : q);   // branch-free
         return z ? diag : y;
     } else if constexpr (KID == KG) {
-        return exp2_nonpos(-q);
+        return exp2_neg(q, e2t);
+    } else if constexpr (KID == KGW) {
+        return exp2_neg_clamped(q, e2t);
     } else {
+        static_assert(KID == KM || KID == KMW, "scalar kernels only");
         // sqrt(q) as q / sqrt(q); the clamp keeps q = 0 (and a rounded
         // negative q from the norm-expanded distance) finite, s ~ 1e-150
         const double qq = __builtin_fmax(q, 1e-300);
         const double s = qq * rsqrt_refined(qq);
-        return (1.0 + s) * exp2_nonpos(-1.4426950408889634 * s);
+        constexpr double c = 128.0 * 1.4426950408889634;   // e^(-s) = 2^(-s c / 128)
+        const double e = KID == KM ? exp2_neg_c(s, c, e2t) : exp2_neg_c_clamped(s, c, e2t);
+        return __builtin_fma(s, e, e);
     }
 }
 
@@ -182,8 +177,8 @@ __device__ __forceinline__ float kprep(float q, float diag) {
 
 // Coordinate scale of the prepared form above.
 __host__ __device__ inline double kscale(int kid, double p0) {
-    if (kid == KG) return __builtin_sqrt(p0 * 1.4426950408889634);
-    if (kid == KM) return 1.7320508075688772 * p0;
+    if (kbase(kid) == KG) return __builtin_sqrt(p0 * 1.4426950408889634);
+    if (kbase(kid) == KM) return 1.7320508075688772 * p0;
     return 1.0;
 }
 

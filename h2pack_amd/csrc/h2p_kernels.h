@@ -746,7 +746,7 @@ __device__ __forceinline__ void jitm_pass(const BlkTask &t, int pbase, int npts,
                                           double scale, double diag, double ox, double oy, double oz,
                                           const double *__restrict__ xin, double *__restrict__ arena,
                                           JitmRow *rowa, JitmNV *rownv, JitmRow *srcb, double *srcx,
-                                          double *red, bool first) {
+                                          double *red, const double *e2t, bool first) {
     constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
     const int lane = threadIdx.x & 63, w = NW == 4 ? (threadIdx.x >> 6) : 0;
     const int tid = NW == 4 ? threadIdx.x : lane;
@@ -812,7 +812,7 @@ __device__ __forceinline__ void jitm_pass(const BlkTask &t, int pbase, int npts,
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int r = 0; r < NR; r++) {
-                        const double kv = kprep<KID>(q[r], diag);
+                        const double kv = kprep<KID>(q[r], diag, e2t);
                         p[r] += kv * xc;
                         acc[c] += kv * v[r];
                     }
@@ -859,7 +859,7 @@ template <int KID, int NW>
 __device__ __forceinline__ void jitm_task(const BlkTask &t, const double *__restrict__ cx,
                                           long long cstride, double scale, double diag,
                                           const double *__restrict__ xin, double *__restrict__ arena,
-                                          double *lds) {
+                                          double *lds, const double *e2t) {
     constexpr int ROWS = NW == 4 ? H2P_JIT_ROWS : H2P_JITW_ROWS;
     JitmRow *rowa = (JitmRow *)lds;
     JitmNV *rownv = (JitmNV *)(lds + 4 * ROWS);
@@ -872,7 +872,7 @@ __device__ __forceinline__ void jitm_task(const BlkTask &t, const double *__rest
         const bool first = (pb == 0);
         switch ((np + 15) >> 4) {
 #define H2P_JITM_CASE(n) \
-            case n: jitm_pass<KID, n, NW>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rowa, rownv, srcb, srcx, red, first); break;
+            case n: jitm_pass<KID, n, NW>(t, pb, np, cx, cstride, scale, diag, ox, oy, oz, xin, arena, rowa, rownv, srcb, srcx, red, e2t, first); break;
             H2P_JITM_CASE(1) H2P_JITM_CASE(2) H2P_JITM_CASE(3) H2P_JITM_CASE(4)
             H2P_JITM_CASE(5) H2P_JITM_CASE(6) H2P_JITM_CASE(7) H2P_JITM_CASE(8)
 #undef H2P_JITM_CASE
@@ -884,19 +884,23 @@ __device__ __forceinline__ void jitm_task(const BlkTask &t, const double *__rest
 
 // Launch unit u of a tiled launch: u < nbig runs task u on the whole
 // workgroup, later units run tasks nbig + 4 (u - nbig) + w, one per wave.
+// The exp2_neg table is copied into LDS first (one copy per workgroup).
 template <int KID>
 __device__ __forceinline__ void blocks_jitm_unit(const BlkTask *__restrict__ tasks, int u, int nbig, int ntask,
                                                  const double *__restrict__ cx, long long cstride,
                                                  double scale, double diag, const double *__restrict__ xin,
                                                  double *__restrict__ arena) {
     __shared__ double lds[H2P_JITM_LDS];
+    __shared__ double e2t[H2P_E2_N];
+    if (threadIdx.x < H2P_E2_N) e2t[threadIdx.x] = c_e2tab[threadIdx.x];
+    __syncthreads();
     if (u < nbig) {
-        jitm_task<KID, 4>(tasks[u], cx, cstride, scale, diag, xin, arena, lds);
+        jitm_task<KID, 4>(tasks[u], cx, cstride, scale, diag, xin, arena, lds, e2t);
     } else {
         const int w = threadIdx.x >> 6;
         const int i = nbig + 4 * (u - nbig) + w;
         if (i < ntask)
-            jitm_task<KID, 1>(tasks[i], cx, cstride, scale, diag, xin, arena, lds + w * H2P_JITM_LDS_WAVE);
+            jitm_task<KID, 1>(tasks[i], cx, cstride, scale, diag, xin, arena, lds + w * H2P_JITM_LDS_WAVE, e2t);
     }
 }
 
@@ -1097,7 +1101,7 @@ __device__ __forceinline__ void blocks_jit2_unit(const BlkTask *__restrict__ tas
                                                  double *__restrict__ arena) {
     if constexpr (sizeof(T) == 4) {
         blocks_jitf_unit<KID>(tasks, u, nbig, ntask, cx, cstride, scale, diag, xin, arena);
-    } else if constexpr (KID == KG || KID == KM) {
+    } else if constexpr (KID == KG || KID == KM || KID == KGW || KID == KMW) {
         blocks_jitm_unit<KID>(tasks, u, nbig, ntask, cx, cstride, scale, diag, xin, arena);
     } else {
         static_assert(KID == KC, "scalar kernels only");
