@@ -20,7 +20,7 @@ HOST_SO = os.path.join(PKG, "libh2p_host.so")
 HIP_SO = os.path.join(PKG, "libh2p_hip.so")
 
 HIP_SOURCES = ["h2p_hip.hip"]
-HIP_HEADERS = ["h2p_kernels.h", "h2p_device.h"]
+HIP_HEADERS = ["h2p_kernels.h", "h2p_device.h", "h2p_exp2.h", "h2p_mm.h", "h2p_batch.h"]
 
 # Packed-FP32 instructions stay disabled: hipcc -O3 would otherwise form
 # v_pk_{add,mul,fma}_f32 from ordinary float code.  MFMA results go to
@@ -55,9 +55,36 @@ def build_hip() -> str:
     return HIP_SO
 
 
+# The H2Pack_p binding (h2pack_amd/glue/h2p_hip_h2pack.c) compiled against
+# the reference's struct H2Pack, with the test helper that fills one
+# (tests/h2pack_struct.c).  H2Pack_config.h pulls in the BLAS/LAPACKE and
+# ASTER headers, which this image lacks and the struct does not use: its
+# include guard is predefined and the one macro the struct needs (DTYPE) is
+# given on the command line; the standard headers the reference headers rely
+# on are included first.  Built only where the reference sources exist.
+REF_SRC = os.environ.get("H2PACK_REF_SRC", "/root/reference/src")
+H2PACK_CHECK_SO = os.path.join(ROOT, "tests", "_build", "libh2p_h2pack_check.so")
+
+
+def build_h2pack_check():
+    if not os.path.exists(os.path.join(REF_SRC, "H2Pack_typedef.h")):
+        return None
+    os.makedirs(os.path.dirname(H2PACK_CHECK_SO), exist_ok=True)
+    _run(["gcc", "-O1", "-fPIC", "-shared", "-Wall", "-Werror=implicit-function-declaration",
+          "-D__H2PACK_CONFIG_H__", "-DDTYPE=double",
+          "-include", "stddef.h", "-include", "stdio.h", "-include", "stdlib.h", "-include", "string.h",
+          "-I", REF_SRC, "-I", os.path.join(ROOT, "include"),
+          "-o", H2PACK_CHECK_SO + ".tmp",
+          os.path.join(PKG, "glue", "h2p_hip_h2pack.c"), os.path.join(ROOT, "tests", "h2pack_struct.c"),
+          "-L", PKG, "-lh2p_hip", "-Wl,-rpath,$ORIGIN/../../h2pack_amd"])
+    os.replace(H2PACK_CHECK_SO + ".tmp", H2PACK_CHECK_SO)
+    return H2PACK_CHECK_SO
+
+
 def build_all() -> None:
     build_host()
     build_hip()
+    build_h2pack_check()
 
 
 if __name__ == "__main__":
@@ -66,3 +93,5 @@ if __name__ == "__main__":
         print(build_host())
     if "hip" in what:
         print(build_hip())
+    if "h2pack" in what:
+        print(build_h2pack_check())

@@ -116,6 +116,9 @@ struct h2p_hip_handle {
     int panel_nvp = 0;
     int mm_var[4][3] = {};        // tile variant per matmul phase and panel width class
     int mv_var = 0;               // variant of the matvec block launch (0 default, 1 other occupancy)
+    // FP64 Gaussian / Matern JIT: the point extent can push the exponential's
+    // argument past the unclamped range, so the launches use KGW / KMW
+    bool wide = false;
     // periodic mode: shift table (x, y, z per entry; entry 0 is zero) and the
     // root's periodic block (also the matmul's dense source for the root)
     int periodic = 0;
@@ -208,6 +211,11 @@ template <int KID, typename T, int PART>
 static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s, const h2p_hip_handle::BlkPart &P) {
     if (P.n == 0) return;
     const BlkTask *tasks = h->d_blk + P.off;
+    if constexpr (KID == KGW || KID == KMW) {   // one build for the clamped kernels
+        hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, 4, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
+                           h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+        return;
+    }
     if (h->mv_var == 1)
         hipLaunchKernelGGL((k_blocks_jit2<KID, T, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
                            h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
@@ -227,8 +235,16 @@ static void dispatch_jit(h2p_hip_handle *h, hipStream_t s, int part) {
     const bool f32 = h->prec == H2P_HIP_FP32MIX;
     switch (h->kid) {
         case KC: f32 ? launch_jit2<KC, float>(h, s, part) : launch_jit2<KC, double>(h, s, part); break;
-        case KG: f32 ? launch_jit2<KG, float>(h, s, part) : launch_jit2<KG, double>(h, s, part); break;
-        case KM: f32 ? launch_jit2<KM, float>(h, s, part) : launch_jit2<KM, double>(h, s, part); break;
+        case KG:
+            if (f32) launch_jit2<KG, float>(h, s, part);
+            else if (h->wide) launch_jit2<KGW, double>(h, s, part);
+            else launch_jit2<KG, double>(h, s, part);
+            break;
+        case KM:
+            if (f32) launch_jit2<KM, float>(h, s, part);
+            else if (h->wide) launch_jit2<KMW, double>(h, s, part);
+            else launch_jit2<KM, double>(h, s, part);
+            break;
         default:
             if (part == 0) launch_jit<KR, 3>(h, s);
             break;
@@ -596,6 +612,23 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     }
     if (rcp_row)
         for (long long i = 0; i < CT; i++) cu[(size_t)4 * CT + i] = 1.0 / cu[(size_t)3 * CT + i];
+    // FP64 Gaussian / Matern JIT: the largest scaled distance bounds the
+    // argument of the exponential; past 2^30 the launches clamp it (wide)
+    if (mode == H2P_HIP_JIT && prec == H2P_HIP_FP64 && (krnl->id == KG || krnl->id == KM)) {
+        double ext = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double *c = cu.data() + (size_t)d * CT;
+            double lo = c[0], hi = c[0];
+            for (long long i = 1; i < CT; i++) {
+                lo = std::min(lo, c[i]);
+                hi = std::max(hi, c[i]);
+            }
+            ext = std::max(ext, hi - lo);
+        }
+        const double rmax = 1.7320508075688772 * ext * kscale(krnl->id, h->kp.p0);
+        const double arg = krnl->id == KG ? rmax * rmax : rmax * 1.4426950408889634;
+        h->wide = !(arg < 0x1p30);   // also for non-finite coordinates or parameters
+    }
     // ---- upward sweep tasks per level
     std::vector<FwdTask> fwdt;
     for (int lv = g->max_level; lv >= g->min_adm_level; lv--) {
@@ -635,6 +668,27 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     double B_pairs = 0, D_pairs = 0;
     const bool per = g->periodic != 0;
     h->periodic = per ? 1 : 0;
+    // pair and leaf indices in range, and node_n_r_adm as H2P_build counts it
+    // from the pair list (src/H2Pack_build.c:707-717: both nodes of a pair,
+    // only node 0 in periodic mode), since it marks the nodes whose skeleton
+    // sums the downward sweep reads
+    {
+        auto node_ok = [&](int v) { return v >= 0 && v < nn; };
+        std::vector<int> cnt(nn, 0);
+        for (int i = 0; i < g->n_r_adm_pair; i++) {
+            const int n0 = g->r_adm_pairs[2 * i], n1 = g->r_adm_pairs[2 * i + 1];
+            if (!node_ok(n0) || !node_ok(n1)) return bail(fail(H2P_HIP_EINVAL, "admissible pair index out of range"));
+            cnt[n0]++;
+            if (!per) cnt[n1]++;
+        }
+        for (int i = 0; i < 2 * g->n_r_inadm_pair; i++)
+            if (!node_ok(g->r_inadm_pairs[i])) return bail(fail(H2P_HIP_EINVAL, "inadmissible pair index out of range"));
+        for (int i = 0; i < nlf; i++)
+            if (!node_ok(g->leaf_nodes[i])) return bail(fail(H2P_HIP_EINVAL, "leaf index out of range"));
+        for (int n = 0; n < nn; n++)
+            if (g->node_n_r_adm[n] != cnt[n])
+                return bail(fail(H2P_HIP_EINVAL, "node_n_r_adm does not match r_adm_pairs"));
+    }
     // shift table: entry 0 is the zero shift, then one entry per distinct offset
     std::vector<double> shtab{0.0, 0.0, 0.0};
     auto shift_index = [&](const double *v) {
@@ -1813,11 +1867,11 @@ static void launch_blocks_b(const BLaunch &b, const char *dbase, const ReqCtx *r
             break;
         case KG:
             if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KG, float, 4>), grid, blk, 0, s, it, b.n_items, reqs);
-            else hipLaunchKernelGGL((k_blocks_jit2_b<KG, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KGW, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
         case KM:
             if (f32) hipLaunchKernelGGL((k_blocks_jit2_b<KM, float, 4>), grid, blk, 0, s, it, b.n_items, reqs);
-            else hipLaunchKernelGGL((k_blocks_jit2_b<KM, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit2_b<KMW, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
         default:
             hipLaunchKernelGGL((k_blocks_jit_b<KR, 3>), grid, blk, 0, s, it, b.n_items, reqs);

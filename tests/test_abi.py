@@ -8,6 +8,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from h2pack_amd import _build, abi
 
@@ -74,3 +75,26 @@ def test_upload_rejects_inconsistent_representation():
     kd = h2mat.kernel_desc(h2.krnl_id, h2.krnl_param)
     rc = lib.h2p_hip_upload(ct.byref(d), ct.byref(kd), abi.JIT, abi.FP64, ct.byref(out))
     assert rc == abi.EINVAL and not out.value
+
+
+def test_upload_validation_rejects_inconsistent_pair_data():
+    """Upload validation (run without the device by h2p_hip_footprint):
+    node_n_r_adm must be the per-node count of r_adm_pairs, as H2P_build
+    computes it, and pair indices must name nodes; otherwise EINVAL."""
+    from h2pack_amd.h2mat import H2PHipError, footprint
+    from tests.problems import problem
+    h2, _ = problem("gaussian")
+    base = footprint(h2)
+    cnt, pairs = h2.node_n_r_adm.copy(), h2.r_adm_pairs.copy()
+    try:
+        h2.node_n_r_adm[int(np.argmax(cnt))] += 1
+        with pytest.raises(H2PHipError, match="node_n_r_adm"):
+            footprint(h2)
+        h2.node_n_r_adm[:] = cnt
+        h2.r_adm_pairs[3] = h2.n_node + 5
+        with pytest.raises(H2PHipError, match="out of range"):
+            footprint(h2)
+    finally:
+        h2.node_n_r_adm[:] = cnt
+        h2.r_adm_pairs[:] = pairs
+    assert footprint(h2) == base
