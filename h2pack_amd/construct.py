@@ -488,9 +488,11 @@ def _calc_reduced_adm_pairs_per(h2: H2Pack) -> None:
 
 
 def rpy_ewald_eval(c0: np.ndarray, c1: np.ndarray, L: float, xi: float, nr: int, nk: int) -> np.ndarray:
-    """Periodic RPY kernel block by Ewald summation (``h2p_host_rpy_ewald``,
-    restating RPY_Ewald_eval_std, src/H2Pack_3D_kernels.h:961-1210); c0, c1
-    are (4, n) with the radius in row 3.  No 1/(6 pi eta) factor."""
+    """Periodic RPY kernel block by Ewald summation (``h2p_host_rpy_ewald``:
+    Beenakker's sum with unequal-radius overlap corrections, the role of
+    RPY_Ewald_eval_std, src/H2Pack_3D_kernels.h:961-1210); c0, c1 are (4, n)
+    with the radius in row 3; real-space images |n_i| <= nr, reciprocal
+    vectors |h_i| <= nk.  No 1/(6 pi eta) factor."""
     from ._lib import host_lib
     c0 = np.ascontiguousarray(c0, dtype=np.float64)
     c1 = np.ascontiguousarray(c1, dtype=np.float64)
@@ -499,7 +501,7 @@ def rpy_ewald_eval(c0: np.ndarray, c1: np.ndarray, L: float, xi: float, nr: int,
     rc = host_lib().h2p_host_rpy_ewald(c0.ctypes.data, n0, n0, c1.ctypes.data, n1, n1, float(L), float(xi),
                                        int(nr), int(nk), out.ctypes.data, 3 * n1, _nthreads())
     if rc != 0:
-        raise ValueError("bad Ewald parameters (nk must not exceed nr)")
+        raise ValueError("bad Ewald parameters (L > 0, xi > 0, nr >= 0, nk >= 0)")
     return out
 
 

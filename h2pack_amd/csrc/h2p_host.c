@@ -16,6 +16,49 @@
 #define KRNL_MATERN32 2
 #define KRNL_RPY      3
 
+/* Rotne-Prager-Yamakawa mobility of two spheres of radii ai, aj at centre
+ * distance r, as K = f I + g u u^T (u = unit separation) in units of
+ * 1/(6 pi eta), for all three configurations (Zuk, Wajnryb, Mizerski,
+ * Szymczak, J. Fluid Mech. 741, R5, 2014):
+ *   one sphere inside the other (r <= |ai - aj|): the larger sphere's
+ *     self-mobility, f = 1 / max(ai, aj), g = 0;
+ *   partial overlap (|ai - aj| < r <= ai + aj):
+ *     f = (16 r^3 (ai + aj) - ((ai - aj)^2 + 3 r^2)^2) / (32 r^3 ai aj),
+ *     g = 3 ((ai - aj)^2 - r^2)^2 / (32 r^3 ai aj);
+ *   disjoint spheres (r > ai + aj), with b = (ai^2 + aj^2) / r^2:
+ *     f = 3 / (4 r) (1 + b / 3),  g = 3 / (4 r) (1 - b).
+ * The reference kernel is src/H2Pack_3D_kernels.h:893-959. */
+static void rpy_far(double r, double ai, double aj, double *f, double *g)
+{
+    const double b = (ai * ai + aj * aj) / (r * r), c = 0.75 / r;
+    *f = c * (1.0 + b / 3.0);
+    *g = c * (1.0 - b);
+}
+
+static void rpy_mobility(double r, double ai, double aj, double *f, double *g)
+{
+    const double dif = fabs(ai - aj), sum = ai + aj;
+    if (r <= dif) {
+        *f = 1.0 / (ai > aj ? ai : aj);
+        *g = 0.0;
+    } else if (r <= sum) {
+        const double r2 = r * r, d2 = dif * dif, w = 1.0 / (32.0 * r2 * r * ai * aj);
+        const double h = d2 + 3.0 * r2, e = d2 - r2;
+        *f = w * (16.0 * r2 * r * sum - h * h);
+        *g = w * 3.0 * e * e;
+    } else {
+        rpy_far(r, ai, aj, f, g);
+    }
+}
+
+/* out (3x3, row stride ldm) = f I + g u u^T with u = d * inv_r */
+static void put_fg(double *out, long ldm, double f, double g, const double *d, double inv_r)
+{
+    const double u[3] = {d[0] * inv_r, d[1] * inv_r, d[2] * inv_r};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) out[a * ldm + b] = g * u[a] * u[b] + (a == b ? f : 0.0);
+}
+
 static void eval_rows(int kid, const double *param,
                       const double *c0, int ld0, int i0, int i1,
                       const double *c1, int ld1, int n1,
@@ -58,43 +101,17 @@ static void eval_rows(int kid, const double *param,
                 row[j] = (1.0 - t) * exp(t);
             }
         }
-    } else {  /* RPY, 3x3 blocks */
+    } else {  /* RPY, 3x3 blocks: C (f I + g u u^T) */
         const double *a0 = c0 + 3 * (long)ld0, *a1 = c1 + 3 * (long)ld1;
         const double C = 1.0 / (6.0 * M_PI * param[0]);
         for (int i = i0; i < i1; i++) {
-            const double tx = x0[i], ty = y0[i], tz = z0[i], ta = a0[i];
-            double *r0 = mat + 3L * i * ldm, *r1 = r0 + ldm, *r2p = r1 + ldm;
+            double *out = mat + 3L * i * ldm;
             for (int j = 0; j < n1; j++) {
-                double dx = tx - x1[j], dy = ty - y1[j], dz = tz - z1[j];
-                double sa = a1[j];
-                double rr2 = dx * dx + dy * dy + dz * dz;
-                double r = sqrt(rr2);
-                double inv_r = (r == 0.0) ? 0.0 : 1.0 / r;
-                double inv_r2 = inv_r * inv_r;
-                dx *= inv_r; dy *= inv_r; dz *= inv_r;
-                double t1, t2, tmp0, tmp1;
-                double tps = ta + sa, tms = ta - sa;
-                if (r > tps) {
-                    tmp0 = C * 0.75 * inv_r;
-                    tmp1 = (ta * ta + sa * sa) * inv_r2;
-                    t1 = tmp0 * (1.0 + tmp1 / 3.0);
-                    t2 = tmp0 * (1.0 - tmp1);
-                } else if (r > fabs(tms)) {
-                    tmp0 = tms * tms;
-                    tmp1 = (inv_r2 * inv_r * C) / (ta * sa * 32.0);
-                    t1 = tmp0 + 3.0 * rr2;
-                    t1 = tmp1 * (16.0 * rr2 * r * tps - t1 * t1);
-                    t2 = tmp0 - rr2;
-                    t2 = tmp1 * 3.0 * t2 * t2;
-                } else {
-                    t1 = C / (ta > sa ? ta : sa);
-                    t2 = 0.0;
-                }
-                double *k0 = r0 + 3 * j, *k1 = r1 + 3 * j, *k2 = r2p + 3 * j;
-                double u0 = t2 * dx, u1 = t2 * dy, u2 = t2 * dz;
-                k0[0] = u0 * dx + t1; k0[1] = u0 * dy;      k0[2] = u0 * dz;
-                k1[0] = u1 * dx;      k1[1] = u1 * dy + t1; k1[2] = u1 * dz;
-                k2[0] = u2 * dx;      k2[1] = u2 * dy;      k2[2] = u2 * dz + t1;
+                const double d[3] = {x0[i] - x1[j], y0[i] - y1[j], z0[i] - z1[j]};
+                const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                double f, g;
+                rpy_mobility(r, a0[i], a1[j], &f, &g);
+                put_fg(out + 3 * j, ldm, C * f, C * g, d, r > 0.0 ? 1.0 / r : 0.0);
             }
         }
     }
@@ -149,115 +166,119 @@ void h2p_host_sketch(int kid, const double *param,
     }
 }
 
-/* Periodic RPY kernel by Ewald summation (the periodic system kernel of
- * H2P_build_periodic), following RPY_Ewald_init_workbuf / RPY_Ewald_eval_std
- * of src/H2Pack_3D_kernels.h:961-1210: for a cubic cell of side L, splitting
- * parameter xi, real-space images |i| <= nr per axis and reciprocal vectors
- * |k| <= nk per axis (half of them, by symmetry), each 3x3 block is the
- * real-space sum (excluding r = 0) + the reciprocal sum + the overlap
- * correction of the minimum image + the self term for coincident points.
- * Coordinates: 4 rows (x, y, z, radius) of length ld.  Like the reference,
- * the result carries no 1/(6 pi eta) factor. */
+/* Periodic RPY kernel by Ewald summation: the periodic system kernel of
+ * H2P_build_periodic (reference: RPY_Ewald_eval_std, src/H2Pack_3D_kernels.h:
+ * 961-1210), here derived from Beenakker, "Ewald sum of the Rotne-Prager
+ * tensor", J. Chem. Phys. 85, 1581 (1986), with his a^2 replaced by the
+ * pair's a3 = (ai^2 + aj^2) / 2 for unequal radii.  Units of 1/(6 pi eta)
+ * (no such factor, as in the reference), cubic cell of side L, splitting
+ * parameter xi:
+ *
+ *   M(d) = sum_{n in Z^3, |n_i| <= nr, d + nL != 0}  Mr(d + nL)
+ *        + sum_{h in Z^3, h != 0, |h_i| <= nk}  Mk(k = 2 pi h / L) cos(k . d)
+ *        + the self term when the two points coincide (up to a lattice
+ *          vector), or else the overlap correction of the nearest image.
+ *
+ * Real space, with s = xi^2 r^2 and alpha = xi^2 a3 (both dimensionless):
+ *   Mr(r) = A I + B u u^T,
+ *   A = erfc(xi r) 3/(4r) (1 + 2 a3 / (3 r^2))
+ *     + xi e^-s / sqrt(pi) (alpha / s + 14 alpha - 9/2 + (3 - 20 alpha) s + 4 alpha s^2),
+ *   B = erfc(xi r) 3/(4r) (1 - 2 a3 / r^2)
+ *     + xi e^-s / sqrt(pi) (-3 alpha / s + 3/2 - 2 alpha + (16 alpha - 3) s - 4 alpha s^2).
+ * Reciprocal space, y = k^2 / (4 xi^2), V = L^3:
+ *   Mk(k) = (6 pi / (V k^2)) (1 + y + 2 y^2) e^-y (1 - a3 k^2 / 3) (I - k^ k^T);
+ *   h and -h give the same term, so only the half-space h > 0
+ *   (lexicographically) is summed, with weight 2.
+ * Self term (coincident points, a = ai = aj):
+ *   (1/a - (6 - 40/3 xi^2 a^2) xi / sqrt(pi)) I.
+ * Overlap correction: the Ewald sum treats every pair with the disjoint-
+ * sphere form; for the nearest image at distance r < ai + aj the full
+ * three-regime mobility replaces it: + (f - f_far) I + (g - g_far) u u^T.
+ * Two points count as coincident when their nearest-image distance is at
+ * most 1e-12 L.  Coordinates: 4 rows (x, y, z, radius) of length ld. */
+typedef struct { double kx, ky, kz, w, wk; } ewald_k;   /* w = weight, wk = w k^2 / 3 */
+
 int h2p_host_rpy_ewald(const double *c0, int ld0, int n0, const double *c1, int ld1, int n1,
                        double L, double xi, int nr, int nk, double *mat, long ldm, int n_thread)
 {
-    const int r_size = (2 * nr + 1) * (2 * nr + 1) * (2 * nr + 1);
-    const int k_size = ((2 * nk + 1) * (2 * nk + 1) * (2 * nk + 1) - 1) / 2;
-    if (nr < 0 || nk < 0 || k_size > r_size) return -1;   /* the k vectors come from the nr cube */
-    double *rs = (double *)malloc(sizeof(double) * 3 * r_size);
-    double *kk = (double *)malloc(sizeof(double) * 6 * (k_size > 0 ? k_size : 1));
-    if (!rs || !kk) { free(rs); free(kk); return -2; }
-    const double V_inv = 1.0 / (L * L * L), tpl = 2.0 * M_PI / L;
-    /* one walk over the nr cube: every (ix, iy, iz) is a real-space shift,
-     * and the first k_size of them (lexicographic order, all before the
-     * centre when nk = nr) are the reciprocal vectors, as in
-     * RPY_Ewald_init_workbuf */
-    int idx = 0;
-    for (int ix = -nr; ix <= nr; ix++)
-        for (int iy = -nr; iy <= nr; iy++)
-            for (int iz = -nr; iz <= nr; iz++, idx++) {
-                rs[3 * idx] = L * ix; rs[3 * idx + 1] = L * iy; rs[3 * idx + 2] = L * iz;
-                if (idx < k_size) {
-                    const double k = tpl * sqrt((double)(ix * ix + iy * iy + iz * iz));
-                    const double k2 = k * k, q = k2 / (xi * xi);
-                    kk[6 * idx] = k;
-                    kk[6 * idx + 1] = 2.0 * V_inv * (1.0 + 0.25 * q + 0.125 * q * q) * 6.0 * M_PI / k2 * exp(-0.25 * q);
-                    kk[6 * idx + 2] = tpl * ix; kk[6 * idx + 3] = tpl * iy; kk[6 * idx + 4] = tpl * iz;
-                    kk[6 * idx + 5] = 1.0 / k;
-                }
+    if (nr < 0 || nk < 0 || !(L > 0.0) || !(xi > 0.0)) return -1;
+    const int side_k = 2 * nk + 1, side_r = 2 * nr + 1;
+    const int n_k = (side_k * side_k * side_k - 1) / 2, n_r = side_r * side_r * side_r;
+    ewald_k *kv = (ewald_k *)malloc(sizeof(ewald_k) * (n_k > 0 ? n_k : 1));
+    double *sh = (double *)malloc(sizeof(double) * 3 * n_r);
+    if (!kv || !sh) { free(kv); free(sh); return -2; }
+    const double two_pi_L = 2.0 * M_PI / L, V = L * L * L;
+    int m = 0;
+    for (int hx = 0; hx <= nk; hx++)           /* h > 0: hx > 0, or hx = 0 and (hy, hz) > 0 */
+        for (int hy = (hx > 0 ? -nk : 0); hy <= nk; hy++)
+            for (int hz = (hx > 0 || hy > 0 ? -nk : 1); hz <= nk; hz++) {
+                const double kx = two_pi_L * hx, ky = two_pi_L * hy, kz = two_pi_L * hz;
+                const double k2 = kx * kx + ky * ky + kz * kz, y = k2 / (4.0 * xi * xi);
+                const double w = 2.0 * (6.0 * M_PI / (V * k2)) * (1.0 + y + 2.0 * y * y) * exp(-y);
+                kv[m++] = (ewald_k){kx, ky, kz, w, w * k2 / 3.0};
             }
-    const double xi2 = xi * xi, xi3 = xi2 * xi, xi5 = xi3 * xi2, xi7 = xi5 * xi2;
-    const double isp = 1.0 / sqrt(M_PI);
-    const double *x0 = c0, *y0 = c0 + ld0, *z0 = c0 + 2 * (long)ld0, *a0 = c0 + 3 * (long)ld0;
-    const double *x1 = c1, *y1 = c1 + ld1, *z1 = c1 + 2 * (long)ld1, *a1 = c1 + 3 * (long)ld1;
+    m = 0;
+    for (int nx = -nr; nx <= nr; nx++)
+        for (int ny = -nr; ny <= nr; ny++)
+            for (int nz = -nr; nz <= nr; nz++, m++) {
+                sh[3 * m] = L * nx;
+                sh[3 * m + 1] = L * ny;
+                sh[3 * m + 2] = L * nz;
+            }
+    const double xi2 = xi * xi, xsp = xi / sqrt(M_PI), coinc2 = 1e-24 * L * L;
+    const double *x0 = c0, *y0 = c0 + ld0, *z0 = c0 + 2 * (long)ld0, *r0 = c0 + 3 * (long)ld0;
+    const double *x1 = c1, *y1 = c1 + ld1, *z1 = c1 + 2 * (long)ld1, *r1 = c1 + 3 * (long)ld1;
     #pragma omp parallel for schedule(dynamic) num_threads(n_thread > 0 ? n_thread : 1)
     for (int i = 0; i < n0; i++) {
-        const double ai = a0[i];
-        const double self_t = 1.0 / ai - (6.0 - 40.0 / 3.0 * xi2 * ai * ai) * xi * isp;
         for (int j = 0; j < n1; j++) {
-            double m00 = 0, m10 = 0, m20 = 0, m11 = 0, m21 = 0, m22 = 0;
-            const double dx = x0[i] - x1[j], dy = y0[i] - y1[j], dz = z0[i] - z1[j], aj = a1[j];
-            const double a3 = 0.5 * (ai * ai + aj * aj);
-            for (int s = 0; s < r_size; s++) {               /* real-space sum */
-                double rx = dx + rs[3 * s], ry = dy + rs[3 * s + 1], rz = dz + rs[3 * s + 2];
-                const double r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 == 0.0) continue;
-                const double r = sqrt(r2), ri = 1.0 / r, ri2 = ri * ri, ri3 = ri * ri2, r4 = r2 * r2;
-                const double e = erfc(xi * r), g = isp * exp(-xi2 * r2);
-                const double t0 = 0.75 * ri, t1 = 0.5 * ri3 * a3, t2 = 4.0 * xi7 * a3 * r4;
-                const double t3 = 3.0 * xi3 * r2, t4 = 4.0 * xi5 * a3 * r2, t5 = 2.0 * xi3 * a3;
-                const double t6 = xi * a3 * ri2;
-                const double p = (t0 + t1) * e + (t2 + t3 - 5.0 * t4 - 4.5 * xi + 7.0 * t5 + t6) * g;
-                const double q = (t0 - 3.0 * t1) * e + (-t2 - t3 + 4.0 * t4 + 1.5 * xi - t5 - 3.0 * t6) * g;
-                rx *= ri; ry *= ri; rz *= ri;
-                m00 += q * rx * rx + p; m10 += q * rx * ry; m20 += q * rx * rz;
-                m11 += q * ry * ry + p; m21 += q * ry * rz; m22 += q * rz * rz + p;
+            const double ai = r0[i], aj = r1[j], a3 = 0.5 * (ai * ai + aj * aj), alpha = xi2 * a3;
+            const double d[3] = {x0[i] - x1[j], y0[i] - y1[j], z0[i] - z1[j]};
+            double M[3][3] = {{0.0}};
+            /* nearest image */
+            double e[3], e2 = 0.0;
+            for (int c = 0; c < 3; c++) {
+                e[c] = d[c] - L * rint(d[c] / L);
+                e2 += e[c] * e[c];
             }
-            for (int s = 0; s < k_size; s++) {               /* reciprocal sum */
-                const double k = kk[6 * s], m2 = kk[6 * s + 1], ki = kk[6 * s + 5];
-                const double t = m2 * cos(kk[6 * s + 2] * dx + kk[6 * s + 3] * dy + kk[6 * s + 4] * dz) *
-                                 (1.0 - a3 * k * k / 3.0);
-                const double ux = kk[6 * s + 2] * ki, uy = kk[6 * s + 3] * ki, uz = kk[6 * s + 4] * ki;
-                m00 += t * (1.0 - ux * ux); m10 -= t * ux * uy; m20 -= t * ux * uz;
-                m11 += t * (1.0 - uy * uy); m21 -= t * uy * uz; m22 += t * (1.0 - uz * uz);
+            const int coincident = e2 <= coinc2;
+            for (int t = 0; t < n_r; t++) {
+                const double v[3] = {d[0] + sh[3 * t], d[1] + sh[3 * t + 1], d[2] + sh[3 * t + 2]};
+                const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (r2 == 0.0 || (coincident && r2 <= coinc2)) continue;
+                const double r = sqrt(r2), s2 = xi2 * r2, q = 0.75 / r, gs = xsp * exp(-s2), ec = erfc(xi * r);
+                const double A = ec * q * (1.0 + 2.0 * a3 / (3.0 * r2)) +
+                                 gs * (alpha / s2 + 14.0 * alpha - 4.5 + (3.0 - 20.0 * alpha) * s2 + 4.0 * alpha * s2 * s2);
+                const double B = ec * q * (1.0 - 2.0 * a3 / r2) +
+                                 gs * (-3.0 * alpha / s2 + 1.5 - 2.0 * alpha + (16.0 * alpha - 3.0) * s2 - 4.0 * alpha * s2 * s2);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) M[a][b] += B * v[a] * v[b] / r2 + (a == b ? A : 0.0);
             }
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 >= 1e-30) {                               /* overlap correction, minimum image */
-                double rx = fmod(dx + 2 * L, L), ry = fmod(dy + 2 * L, L), rz = fmod(dz + 2 * L, L);
-                rx = rx > 0.5 * L ? rx - L : rx;
-                ry = ry > 0.5 * L ? ry - L : ry;
-                rz = rz > 0.5 * L ? rz - L : rz;
-                const double s2 = rx * rx + ry * ry + rz * rz, r = sqrt(s2), r3 = s2 * r;
-                const double ri = 1.0 / r, ri3 = ri * ri * ri;
-                rx *= ri; ry *= ri; rz *= ri;
-                const double u = (ai * ai + aj * aj) / s2;
-                const double f1 = 0.75 * ri * (1.0 + u / 3.0), f2 = 0.75 * ri * (1.0 - u);
-                const double d = ai - aj;
-                double t1, t2;
-                if (r > ai + aj) {
-                    t1 = f1; t2 = f2;
-                } else if (r > fabs(d)) {
-                    const double w = ri3 / (32.0 * ai * aj);
-                    t1 = d * d + 3.0 * s2;
-                    t1 = (16.0 * r3 * (ai + aj) - t1 * t1) * w;
-                    t2 = d * d - s2;
-                    t2 = 3.0 * t2 * t2 * w;
-                } else {
-                    t1 = 1.0 / (ai > aj ? ai : aj); t2 = 0.0;
+            for (int t = 0; t < n_k; t++) {
+                const ewald_k *k = kv + t;
+                const double k2 = 3.0 * k->wk / k->w, kk[3] = {k->kx, k->ky, k->kz};
+                const double c = (k->w - a3 * k->wk) * cos(k->kx * d[0] + k->ky * d[1] + k->kz * d[2]);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) M[a][b] += c * ((a == b ? 1.0 : 0.0) - kk[a] * kk[b] / k2);
+            }
+            if (coincident) {
+                const double self = 1.0 / ai - (6.0 - 40.0 / 3.0 * xi2 * ai * ai) * xsp;
+                for (int a = 0; a < 3; a++) M[a][a] += self;
+            } else {
+                const double r = sqrt(e2);
+                if (r <= ai + aj) {
+                    double f, g, ff, gf;
+                    rpy_mobility(r, ai, aj, &f, &g);
+                    rpy_far(r, ai, aj, &ff, &gf);
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) M[a][b] += (g - gf) * e[a] * e[b] / e2 + (a == b ? f - ff : 0.0);
                 }
-                t1 -= f1; t2 -= f2;
-                m00 += t2 * rx * rx + t1; m10 += t2 * rx * ry; m20 += t2 * rx * rz;
-                m11 += t2 * ry * ry + t1; m21 += t2 * ry * rz; m22 += t2 * rz * rz + t1;
-            } else {                                         /* self term */
-                m00 += self_t; m11 += self_t; m22 += self_t;
             }
-            double *b = mat + (long)(3 * i) * ldm + 3 * j;
-            b[0] = m00;       b[1] = m10;       b[2] = m20;
-            b[ldm] = m10;     b[ldm + 1] = m11; b[ldm + 2] = m21;
-            b[2 * ldm] = m20; b[2 * ldm + 1] = m21; b[2 * ldm + 2] = m22;
+            double *out = mat + (long)(3 * i) * ldm + 3 * j;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) out[a * ldm + b] = M[a][b];
         }
     }
-    free(rs);
-    free(kk);
+    free(kv);
+    free(sh);
     return 0;
 }

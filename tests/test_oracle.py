@@ -214,9 +214,30 @@ def test_ewald_kernel_invariants():
     K3 = C.rpy_ewald_eval(c, c2, L, 0.3, 6, 6)
     off = np.kron(~np.eye(40, dtype=bool), np.ones((3, 3), dtype=bool))
     assert np.abs(K3 - K2)[off].max() / np.abs(K2).max() < 1e-13
-    # a planted mistake: truncated reciprocal sum breaks the xi independence
-    K4 = C.rpy_ewald_eval(c, c, L, 0.3, 6, 2)
-    assert np.abs(K4 - K2).max() / np.abs(K2).max() > 1e-6
+    # the reciprocal truncation depends on nk alone: with the real-space sum
+    # already converged at nr = 2, nr = 6 changes nothing ...
+    K62 = C.rpy_ewald_eval(c, c, L, 0.3, 6, 2)
+    K22 = C.rpy_ewald_eval(c, c, L, 0.3, 2, 2)
+    assert np.abs(K62 - K22).max() / np.abs(K22).max() < 1e-12
+    # ... while a truncated reciprocal sum (a planted mistake) breaks the
+    # splitting-parameter independence
+    assert np.abs(K62 - K2).max() / np.abs(K2).max() > 1e-6
+
+
+def test_ewald_self_mobility_matches_hasimoto():
+    """One sphere of radius a per cubic cell of side L: the periodic
+    self-mobility is (1/a) (1 - 2.8372974794806 a/L + (4 pi / 3) (a/L)^3)
+    (Hasimoto, J. Fluid Mech. 5, 317 (1959), simple cubic array; the RPY
+    Ewald sum reproduces these terms, Beenakker 1986), isotropic, and
+    independent of where the sphere sits and of the splitting parameter."""
+    H = 2.8372974794806
+    for a, L in ((0.5, 10.0), (1.0, 10.0), (1.0, 6.0), (0.3, 20.0)):
+        c = np.array([[1.3], [2.1], [0.7], [a]])
+        for xi in (2.5 / L, 4.0 / L):
+            K = C.rpy_ewald_eval(c, c, L, xi, 6, 8)
+            want = (1.0 - H * a / L + 4.0 * np.pi / 3.0 * (a / L) ** 3) / a
+            assert abs(K[0, 0] - want) < 1e-12 * want, (a, L, xi, K[0, 0], want)
+            assert np.abs(K - K[0, 0] * np.eye(3)).max() < 1e-13 * want
 
 
 def test_periodic_pairs_are_one_sided_images():
