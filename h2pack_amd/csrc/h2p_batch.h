@@ -153,6 +153,17 @@ __global__ void __launch_bounds__(256) k_mm_b(const BatchItem *__restrict__ item
     mm_body<KID, KD, G>(t, c.srcs, c.U, c.UT, c.BD, c.coord, c.CT, c.kp, c.shifts, c.P);
 }
 
+// The FP32-mixed product phases (coupling, and the B/D part of the leaves)
+// of FP32-mixed handles; BD is then the FP32 block store (AOT).
+template <int KID, class G>
+__global__ void __launch_bounds__(256) k_mm32_b(const BatchItem *__restrict__ items, int n,
+                                                const MMCtx *__restrict__ reqs) {
+    const int it = find_item(items, n, blockIdx.x);
+    const MMCtx &c = reqs[items[it].req];
+    const MMTask t = ((const MMTask *)items[it].tasks)[blockIdx.x - items[it].begin];
+    mm32_body<KID, G>(t, c.srcs, (const float *)c.BD, c.coord, c.CT, (float)c.kp.p0, c.shifts, c.P);
+}
+
 __global__ void __launch_bounds__(256) k_mm_gather_b(const BatchItem *__restrict__ items, int n,
                                                      const MMCtx *__restrict__ reqs) {
     const int it = find_item(items, n, blockIdx.x);

@@ -16,6 +16,7 @@
 namespace h2p {
 
 typedef double d4 __attribute__((ext_vector_type(4)));   // one lane's share of a 16x16 FP64 MFMA tile
+typedef float f4 __attribute__((ext_vector_type(4)));    // one lane's share of a 16x16 FP32 MFMA tile
 
 // Kernel ids.  KGW / KMW are Gaussian / Matern-3/2 for FP64 JIT handles
 // whose point extent can push the exponential's argument past the range of

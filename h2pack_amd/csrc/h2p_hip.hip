@@ -112,6 +112,11 @@ struct h2p_hip_handle {
     std::vector<MMTask> h_mmt;
     std::vector<MMSrc> h_mms;
     Level mm_intmd{0, 0}, mm_leaf{0, 0};
+    // FP32-mixed matmul (scalar kernels, JIT or FP32-stored AOT): the
+    // coupling phase and the B/D part of the leaf outputs (mm_leaf32, after
+    // the U part mm_leaf) run in the FP32 product kernel
+    bool mm_f32 = false;
+    Level mm_leaf32{0, 0};
     double *d_panels = nullptr;   // PX | Y0 | Y1 | PY, panel_nvp wide
     int panel_nvp = 0;
     int mm_var[4][3] = {};        // tile variant per matmul phase and panel width class
@@ -428,19 +433,62 @@ static void dispatch_mm(const h2p_hip_handle *h, const Level &L, const MMPanels 
     }
 }
 
-// the four phase classes of a matmul, as level lists
+template <int KID, int NV, int V>
+static void launch_mm32_v(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s) {
+    hipLaunchKernelGGL((k_mm32<KID, MMVar<1, NV, V>>), dim3(L.n), dim3(256), 0, s, h->d_mmt + L.off, h->d_mms,
+                       (const float *)h->d_blob, h->d_coord, h->CT, (float)h->kp.p0, h->d_shift, P);
+}
+
+template <int KID, int NV>
+static void launch_mm32_nv(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s, int v) {
+    switch (v) {
+        case 1: launch_mm32_v<KID, NV, 1>(h, L, P, s); break;
+        case 2: launch_mm32_v<KID, NV, 2>(h, L, P, s); break;
+        case 3: launch_mm32_v<KID, NV, 3>(h, L, P, s); break;
+        default: launch_mm32_v<KID, NV, 0>(h, L, P, s); break;
+    }
+}
+
+template <int KID>
+static void launch_mm32(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s, int v) {
+    switch (nv_class(P.nvp)) {
+        case 0: launch_mm32_nv<KID, 16>(h, L, P, s, v); break;
+        case 1: launch_mm32_nv<KID, 32>(h, L, P, s, v); break;
+        default: launch_mm32_nv<KID, 64>(h, L, P, s, v); break;
+    }
+}
+
+static void dispatch_mm32(const h2p_hip_handle *h, const Level &L, const MMPanels &P, hipStream_t s, int v) {
+    if (L.n <= 0) return;
+    switch (h->kid) {
+        case KC: launch_mm32<KC>(h, L, P, s, v); break;
+        case KG: launch_mm32<KG>(h, L, P, s, v); break;
+        default: launch_mm32<KM>(h, L, P, s, v); break;
+    }
+}
+
+// the four phase classes of a matmul, as level lists for the FP64 product
+// kernel and for the FP32 one (FP32-mixed handles)
 enum { MMP_FWD = 0, MMP_INTMD = 1, MMP_BWD = 2, MMP_LEAF = 3 };
 static std::vector<Level> mm_phase_levels(const h2p_hip_handle *h, int ph) {
     switch (ph) {
         case MMP_FWD: return h->mm_fwd_levels;
-        case MMP_INTMD: return {h->mm_intmd};
+        case MMP_INTMD: return h->mm_f32 ? std::vector<Level>{} : std::vector<Level>{h->mm_intmd};
         case MMP_BWD: return h->mm_bwd_levels;
         default: return {h->mm_leaf};
     }
 }
 
+static std::vector<Level> mm_phase_levels32(const h2p_hip_handle *h, int ph) {
+    if (!h->mm_f32) return {};
+    if (ph == MMP_INTMD) return {h->mm_intmd};
+    if (ph == MMP_LEAF) return {h->mm_leaf32};
+    return {};
+}
+
 static void launch_mm_phase(const h2p_hip_handle *h, int ph, const MMPanels &P, hipStream_t s, int v) {
     for (const Level &L : mm_phase_levels(h, ph)) dispatch_mm(h, L, P, s, v);
+    for (const Level &L : mm_phase_levels32(h, ph)) dispatch_mm32(h, L, P, s, v);
 }
 
 static void drop_mm_graphs(h2p_hip_handle *h) {
@@ -887,6 +935,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     std::vector<MMTask> mmt;
     std::vector<MMSrc> mms;
     const bool jit = mode == H2P_HIP_JIT;
+    const bool mm_f32 = prec == H2P_HIP_FP32MIX && krnl->id != KR && !per;
+    h->mm_f32 = mm_f32;
     {
         // per-destination source lists, in pair order
         std::vector<std::vector<MMSrc>> y1_src(nn), leaf_src(nn);
@@ -1009,7 +1059,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             L.n = (int)mmt.size() - L.off;
             if (L.n > 0) h->mm_bwd_levels.push_back(L);
         }
-        // leaf outputs
+        // leaf outputs; FP32-mixed handles split them into the U part (FP64
+        // kernel, first writer) and the B/D part (FP32 kernel, accumulating)
         {
             Level L{(int)mmt.size(), 0};
             for (int i = 0; i < nlf; i++) {
@@ -1021,12 +1072,23 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                     a.x_space = MM_Y1; a.x_off = soff[n];
                     src.push_back(a);
                 }
-                for (const MMSrc &x : leaf_src[n]) src.push_back(x);
+                if (!mm_f32)
+                    for (const MMSrc &x : leaf_src[n]) src.push_back(x);
                 add_task(MM_PY, g->mat_cluster[2 * n], npt(n) * kd, 0, src);
             }
             L.n = (int)mmt.size() - L.off;
             std::stable_sort(mmt.begin() + L.off, mmt.end(), [&](const MMTask &x, const MMTask &y) { return work(x) > work(y); });
             h->mm_leaf = L;
+            if (mm_f32) {
+                Level L32{(int)mmt.size(), 0};
+                for (int i = 0; i < nlf; i++) {
+                    int n = g->leaf_nodes[i];
+                    if (!leaf_src[n].empty()) add_task(MM_PY, g->mat_cluster[2 * n], npt(n) * kd, 1, leaf_src[n]);
+                }
+                L32.n = (int)mmt.size() - L32.off;
+                std::stable_sort(mmt.begin() + L32.off, mmt.end(), [&](const MMTask &x, const MMTask &y) { return work(x) > work(y); });
+                h->mm_leaf32 = L32;
+            }
         }
     }
     // ---- every access of every task stays inside its buffer
@@ -1137,7 +1199,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     h->h_mmt = std::move(mmt);
     h->h_mms = std::move(mms);
     h->h_blk = blk;
-    h->mm_ok = !(mode == H2P_HIP_AOT && prec == H2P_HIP_FP32MIX);
+    h->mm_ok = !(mode == H2P_HIP_AOT && prec == H2P_HIP_FP32MIX && !mm_f32);
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_yp, (size_t)S, acct))) return bail(rc);
@@ -1364,7 +1426,9 @@ static std::string tune_signature(const h2p_hip_handle *h, int ph, int cls, cons
                                   const std::vector<MMSrc> &srcs) {
     // histogram of (log2 M, log2 K) over the phase's (task, source) products
     std::map<std::pair<int, int>, long long> hist;
-    for (const Level &L : mm_phase_levels(h, ph))
+    std::vector<Level> lv = mm_phase_levels(h, ph), lv32 = mm_phase_levels32(h, ph);
+    lv.insert(lv.end(), lv32.begin(), lv32.end());
+    for (const Level &L : lv)
         for (int i = L.off; i < L.off + L.n; i++) {
             const MMTask &t = tasks[i];
             for (int k = t.sbeg; k < t.send; k++) {
@@ -1374,7 +1438,7 @@ static std::string tune_signature(const h2p_hip_handle *h, int ph, int cls, cons
             }
         }
     std::string sig = std::to_string(h->kid) + "/" + std::to_string(h->mode) + "/" + std::to_string(ph) + "/" +
-                      std::to_string(cls) + ":";
+                      std::to_string(cls) + (h->mm_f32 ? "/f32" : "") + ":";
     for (const auto &e : hist) {
         // counts on a log2 scale, so nearby histograms share a choice
         const int c = (int)std::floor(std::log2((double)e.second));
@@ -1541,6 +1605,7 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
     for (int ph = 0; ph < 4 && !err; ph++) {
         bool empty = true;
         for (const Level &L : mm_phase_levels(h, ph)) empty = empty && L.n <= 0;
+        for (const Level &L : mm_phase_levels32(h, ph)) empty = empty && L.n <= 0;
         if (empty) continue;
         const std::string sig = tune_signature(h, ph, cls, h->h_mmt, h->h_mms);
         double t[H2P_MM_NVARIANT];
@@ -2099,29 +2164,33 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
         }
         const BLaunch lg = finish(tb, g), ls = finish(tb, sc);
         // product phases, grouped by kernel id: (level list, launch)
-        struct MMLaunch { BLaunch b; int kid; };
+        // (FP32-mixed handles run their coupling phase and the B/D part of
+        // their leaf outputs in the FP32 kernel: grouped apart)
+        struct MMLaunch { BLaunch b; int kid; bool f32; };
         std::vector<MMLaunch> phases;
-        auto add_phase = [&](auto level_of) {
+        auto add_phase = [&](auto level_of, bool coupling) {
             std::vector<int> used(m, 0);
             for (int i = 0; i < m; i++) {
                 if (used[i]) continue;
+                const bool f32 = coupling && act[i]->mm_f32;
                 ItemList L;
                 for (int j = i; j < m; j++) {
-                    if (used[j] || act[j]->kid != act[i]->kid) continue;
+                    if (used[j] || act[j]->kid != act[i]->kid || (coupling && act[j]->mm_f32) != f32) continue;
                     used[j] = 1;
                     const Level *lv = level_of(act[j]);
                     if (lv) L.add(j, act[j]->d_mmt + lv->off, lv->n);
                 }
                 BLaunch b = finish(tb, L);
-                if (b.grid > 0) phases.push_back({b, act[i]->kid});
+                if (b.grid > 0) phases.push_back({b, act[i]->kid, f32});
             }
         };
         for (size_t k = 0; k < max_fwd; k++)
-            add_phase([&](h2p_hip_handle *h) { return k < h->mm_fwd_levels.size() ? &h->mm_fwd_levels[k] : (const Level *)nullptr; });
-        add_phase([&](h2p_hip_handle *h) { return (const Level *)&h->mm_intmd; });
+            add_phase([&](h2p_hip_handle *h) { return k < h->mm_fwd_levels.size() ? &h->mm_fwd_levels[k] : (const Level *)nullptr; }, false);
+        add_phase([&](h2p_hip_handle *h) { return (const Level *)&h->mm_intmd; }, true);
         for (size_t k = 0; k < max_bwd; k++)
-            add_phase([&](h2p_hip_handle *h) { return k < h->mm_bwd_levels.size() ? &h->mm_bwd_levels[k] : (const Level *)nullptr; });
-        add_phase([&](h2p_hip_handle *h) { return (const Level *)&h->mm_leaf; });
+            add_phase([&](h2p_hip_handle *h) { return k < h->mm_bwd_levels.size() ? &h->mm_bwd_levels[k] : (const Level *)nullptr; }, false);
+        add_phase([&](h2p_hip_handle *h) { return (const Level *)&h->mm_leaf; }, false);
+        add_phase([&](h2p_hip_handle *h) { return h->mm_f32 ? (const Level *)&h->mm_leaf32 : (const Level *)nullptr; }, true);
         const char *dbase = nullptr;
         if ((rc = stage_tables(B, tb, s, &dbase))) return rc;
         const MMCtx *dctx = (const MMCtx *)(dbase + ctx_off);
@@ -2134,6 +2203,20 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
         for (const MMLaunch &ph : phases) {
             const BatchItem *it = (const BatchItem *)(dbase + ph.b.items_off);
             const dim3 grid(ph.b.grid), blk(256);
+            if (ph.f32) {
+                switch (ph.kid * 3 + ncls) {
+                    case KC * 3 + 0: hipLaunchKernelGGL((k_mm32_b<KC, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KC * 3 + 1: hipLaunchKernelGGL((k_mm32_b<KC, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KC * 3 + 2: hipLaunchKernelGGL((k_mm32_b<KC, MMVar<1, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KG * 3 + 0: hipLaunchKernelGGL((k_mm32_b<KG, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KG * 3 + 1: hipLaunchKernelGGL((k_mm32_b<KG, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KG * 3 + 2: hipLaunchKernelGGL((k_mm32_b<KG, MMVar<1, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KM * 3 + 0: hipLaunchKernelGGL((k_mm32_b<KM, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    case KM * 3 + 1: hipLaunchKernelGGL((k_mm32_b<KM, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                    default: hipLaunchKernelGGL((k_mm32_b<KM, MMVar<1, 64, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
+                }
+                continue;
+            }
             switch (ph.kid * 3 + ncls) {
                 case KC * 3 + 0: hipLaunchKernelGGL((k_mm_b<KC, 1, MMVar<1, 16, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;
                 case KC * 3 + 1: hipLaunchKernelGGL((k_mm_b<KC, 1, MMVar<1, 32, 0>>), grid, blk, 0, s, it, ph.b.n_items, dctx); break;

@@ -275,6 +275,174 @@ __global__ void __launch_bounds__(256) k_mm(const MMTask *__restrict__ tasks,
     mm_body<KID, KD, G>(tasks[blockIdx.x], srcs, Ublob, UTblob, BDblob, cx, cstride, kp, shifts, P);
 }
 
+// ---------------------------------------------------------------------------
+// FP32-mixed products (handles of precision FP32MIX): the B and D sources
+// -- kernel values evaluated in FP32 (JIT: the FP64 coordinate difference and
+// squared distance rounded once to FP32, then the hardware exp2 / rsqrt as
+// in the FP32-mixed matvec) or blocks stored in FP32 (AOT) -- and their
+// operand rows rounded to FP32 run on v_mfma_f32_16x16x4f32; the FP32 tile
+// sums of every 16 inner entries (four products, counted from the start of
+// each source) are folded into FP64 accumulators, and the output panels
+// stay FP64.  The fold points do not depend on the tile variant (KC is a
+// multiple of 16), so every variant gives bitwise the same result.  The U phases keep the FP64 kernel.  Only
+// B/D sources reach this kernel (the host splits the leaf tasks into a U
+// part and a B/D part).
+//
+// Lane layouts: v_mfma_f32_16x16x4f32 takes A[l % 16][l / 16] and
+// B[l / 16][l % 16] like the FP64 product, but returns D[4 (l / 16) + r]
+// [l % 16] in register r, where the FP64 product returns row 4 r + l / 16.
+// Feeding the A rows in the order sigma(4 a + b) = 4 b + a makes the FP32
+// tile land in the FP64 accumulator layout, so the fold is lane-local.
+__device__ __forceinline__ int mm32_sigma(int i) { return 4 * (i & 3) + (i >> 2); }
+
+template <int KID, class G>
+struct MMStage32 {
+    static constexpr int RC = G::RC, KC = G::KC, NV = G::NV;
+    static_assert(KC % 16 == 0, "FP32 stages fold every 16 inner entries");
+    static constexpr int NA = (RC * KC + 255) / 256;
+    static constexpr int NX = (KC * NV + 255) / 256;
+    float a[NA];
+    float x[NX];
+
+    __device__ __forceinline__ void fetch(const MMTask &t, const MMSrc &s, int mc, int kc, int nvp,
+                                          const float *__restrict__ BD32, const double *__restrict__ cx,
+                                          long long cstride, float p0, const double *__restrict__ shifts,
+                                          const MMPanels &P) {
+        const double *cy = cx + cstride, *cz = cx + 2 * cstride;
+        if (s.kind == MM_A_JIT) {
+            const int np0 = t.M - mc, nq0 = s.K - kc;
+            const double hx = shifts[3 * s.shift], hy = shifts[3 * s.shift + 1], hz = shifts[3 * s.shift + 2];
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                const int e = threadIdx.x + 256 * u;
+                const int i = e / KC, j = e - i * KC;
+                const bool ok = e < RC * KC && i < np0 && j < nq0;
+                const int gi = s.c0 + mc + (ok ? i : 0), gj = s.c1 + kc + (ok ? j : 0);
+                const double dx = cx[gi] - (cx[gj] + hx), dy = cy[gi] - (cy[gj] + hy), dz = cz[gi] - (cz[gj] + hz);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                a[u] = ok ? kvalf<KID>((float)r2, p0) : 0.0f;
+            }
+        } else {
+            const float *A = BD32 + s.a_off;
+#pragma unroll
+            for (int u = 0; u < NA; u++) {
+                const int e = threadIdx.x + 256 * u;
+                int i, j;
+                if (!s.trans) { i = e / KC; j = e - i * KC; }
+                else { j = e / RC; i = e - j * RC; }
+                const int m = mc + i, k = kc + j;
+                const bool ok = e < RC * KC && m < t.M && k < s.K;
+                a[u] = ok ? (s.trans ? A[(long long)k * s.lda + m] : A[(long long)m * s.lda + k]) : 0.0f;
+            }
+        }
+        const double *X = P.p[s.x_space] + (long long)s.x_off * nvp;
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int e = threadIdx.x + 256 * u;
+            const int k = e / nvp, c = e - k * nvp;
+            x[u] = (e < KC * nvp && kc + k < s.K) ? (float)X[(long long)(kc + k) * nvp + c] : 0.0f;
+        }
+    }
+
+    __device__ __forceinline__ void store(const MMSrc &s, int nvp, float (*As)[KC + 1], float (*Xs)[NV + 1]) const {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = threadIdx.x + 256 * u;
+            if (e >= RC * KC) break;
+            int i, j;
+            if (s.kind == MM_A_JIT || !s.trans) { i = e / KC; j = e - i * KC; }
+            else { j = e / RC; i = e - j * RC; }
+            As[i][j] = a[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NX; u++) {
+            const int e = threadIdx.x + 256 * u;
+            if (e >= KC * nvp) break;
+            const int k = e / nvp, c = e - k * nvp;
+            Xs[k][c] = x[u];
+        }
+    }
+};
+
+template <int KID, class G>
+__device__ __forceinline__ void mm32_body(const MMTask &t, const MMSrc *__restrict__ srcs,
+                                          const float *__restrict__ BD32, const double *__restrict__ cx,
+                                          long long cstride, float p0, const double *__restrict__ shifts,
+                                          const MMPanels &P) {
+    constexpr int RC = G::RC, KC = G::KC, NV = G::NV;
+    constexpr int MT = RC / 16;
+    constexpr int SLOTS = (MT * (NV / 16) + 3) / 4;
+    __shared__ float As[RC][KC + 1];
+    __shared__ float Xs[KC][NV + 1];
+    const int nvp = P.nvp, NT = nvp >> 4;
+    const int T = MT * NT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lq = lane >> 4, ls = mm32_sigma(li);
+    double *dst = P.p[t.dst_space] + (long long)t.dst_off * nvp;
+    int s_first = t.sbeg;
+    while (s_first < t.send && srcs[s_first].K <= 0) s_first++;
+    MMStage32<KID, G> stg;
+    for (int mc = 0; mc < t.M; mc += RC) {
+        d4 acc[SLOTS];
+#pragma unroll
+        for (int j = 0; j < SLOTS; j++) acc[j] = (d4){0.0, 0.0, 0.0, 0.0};
+        int si = s_first, kc = 0;
+        if (si < t.send) stg.fetch(t, srcs[si], mc, kc, nvp, BD32, cx, cstride, p0, shifts, P);
+        while (si < t.send) {
+            const MMSrc cur = srcs[si];
+            __syncthreads();
+            stg.store(cur, nvp, As, Xs);
+            __syncthreads();
+            int nsi = si, nkc = kc;
+            mm_next(srcs, t.send, KC, nsi, nkc);
+            if (nsi < t.send) stg.fetch(t, srcs[nsi], mc, nkc, nvp, BD32, cx, cstride, p0, shifts, P);
+#pragma unroll
+            for (int j = 0; j < SLOTS; j++) {
+                const int tt = w + 4 * j;
+                if (tt < T) {
+                    const int mt = tt / NT, nt = tt - mt * NT;
+#pragma unroll
+                    for (int k0 = 0; k0 < KC; k0 += 16) {
+                        f4 a32 = (f4){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int kk = k0; kk < k0 + 16; kk += 4)
+                            a32 = __builtin_amdgcn_mfma_f32_16x16x4f32(As[mt * 16 + ls][kk + lq],
+                                                                       Xs[kk + lq][nt * 16 + li], a32, 0, 0, 0);
+#pragma unroll
+                        for (int r = 0; r < 4; r++) acc[j][r] += (double)a32[r];
+                    }
+                }
+            }
+            si = nsi;
+            kc = nkc;
+        }
+#pragma unroll
+        for (int j = 0; j < SLOTS; j++) {
+            const int tt = w + 4 * j;
+            if (tt < T) {
+                const int mt = tt / NT, nt = tt - mt * NT;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m = mc + mt * 16 + 4 * r + lq;
+                    if (m < t.M) {
+                        double *o = dst + (long long)m * nvp + nt * 16 + li;
+                        *o = t.accumulate ? *o + acc[j][r] : acc[j][r];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int KID, class G>
+__global__ void __launch_bounds__(256) k_mm32(const MMTask *__restrict__ tasks, const MMSrc *__restrict__ srcs,
+                                              const float *__restrict__ BD32, const double *__restrict__ cx,
+                                              long long cstride, float p0, const double *__restrict__ shifts,
+                                              MMPanels P) {
+    mm32_body<KID, G>(tasks[blockIdx.x], srcs, BD32, cx, cstride, p0, shifts, P);
+}
+
 // Caller panel (original order, row- or column-major, ld) -> PX (permuted,
 // row-major nvp, zero padding), for columns [v0, v0 + nv).
 __global__ void k_mm_gather(const double *__restrict__ X, long long ldx, int row_major, int v0, int nv,

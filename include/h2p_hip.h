@@ -45,9 +45,14 @@ extern "C" {
 
 /* arithmetic of the B/D phase */
 #define H2P_HIP_FP64    0  /* everything in FP64 */
-#define H2P_HIP_FP32MIX 1  /* FP32 kernel values and block products with FP32
-                              per-tile sums folded into FP64; U sweeps and the
-                              output accumulation stay FP64 */
+#define H2P_HIP_FP32MIX 1  /* FP32 kernel values (JIT) or FP32-stored blocks
+                              (AOT) and FP32 B/D products, their FP32 partial
+                              sums folded into FP64; U sweeps and the output
+                              accumulation stay FP64.  Matvec: all kernels in
+                              AOT, the scalar kernels in JIT (RPY JIT:
+                              H2P_HIP_ENOSYS at upload).  Matmul: the scalar
+                              kernels, JIT and AOT, on FP32 matrix cores
+                              (RPY AOT: H2P_HIP_ENOSYS). */
 
 /* CBLAS_LAYOUT values */
 #define H2P_HIP_ROW_MAJOR 101

@@ -92,7 +92,7 @@ def test_matvec_and_matmul_match_oracle(name):
         y = m.matvec(x)
         print(f"[{name} {mode} {prec}] matvec", flush=True)
         assert _close(y, y_ref, tol), (mode, prec)
-        if m.info()["prec"] == abi.FP64 or mode == "jit":
+        if m.info()["prec"] == abi.FP64 or CASES[name][0] != C.KRNL_RPY:   # FP32-mixed matmul: scalar kernels
             Y = m.matmul(X, "col")
             assert _close(Y, Y_ref, tol), (mode, prec)
         m.close()

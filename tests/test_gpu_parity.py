@@ -145,6 +145,35 @@ def test_rpy_aot_fp32mix_within_build_tolerance():
     assert rel(y, y_ref) < h2.QR_stop_tol
 
 
+@pytest.mark.parametrize("name", ["gaussian", "matern32", "coulomb"])
+@pytest.mark.parametrize("mode", ["jit", "aot"])
+def test_matmul_fp32mix_within_build_tolerance(name, mode):
+    """FP32-mixed H2P_matmul (FP32 kernel values or FP32-stored blocks in
+    the B/D products on FP32 MFMA, folded into FP64; FP64 U phases) against
+    the FP64 oracle, both layouts, 8 and 32 columns; it is not the FP64
+    product in disguise (the results differ from it bitwise)."""
+    h2, _ = problem(name)
+    orc = O.OracleH2(h2, BD_JIT=1)
+    m32 = H2MatHIP(h2, mode=mode, prec="fp32mix", host_blocks=False)
+    m64 = H2MatHIP(h2, mode=mode, host_blocks=False)
+    rng = np.random.default_rng(21)
+    for nv in (8, 32):
+        X = rng.standard_normal((h2.krnl_mat_size, nv))
+        Y_ref = orc.matmul(X, "col", n_thread=8)
+        Yc = m32.matmul(X, "col")
+        Yr = m32.matmul(X, "row")
+        err_c, err_r = rel(Yc, Y_ref), rel(Yr, Y_ref)
+        print(f"[{name} {mode} fp32mix matmul n_vec={nv}] rel err vs FP64 oracle col {err_c:.3e} "
+              f"row {err_r:.3e}", flush=True)
+        assert err_c < h2.QR_stop_tol and err_r < h2.QR_stop_tol
+        assert np.array_equal(Yc, Yr)
+        Y64 = m64.matmul(X, "col")
+        assert not np.array_equal(Yc, Y64)
+        assert rel(Y64, Y_ref) < FP64_TOL
+    m32.close()
+    m64.close()
+
+
 def test_fp32mix_refused_where_unavailable():
     h2, _ = problem("tiny_rpy")
     with pytest.raises(H2PHipError) as ei:

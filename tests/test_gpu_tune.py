@@ -73,3 +73,22 @@ def test_matvec_variants_give_the_same_result(name, mode, prec):
     assert (t2["variants"][0], t2["d_concurrent"]) == (t["variants"][0], t["d_concurrent"])
     assert np.array_equal(m2.matvec(x), y0)
     m2.close()
+
+
+@pytest.mark.parametrize("name,mode", [("gaussian", "jit"), ("coulomb", "aot")])
+def test_fp32mix_variants_give_the_same_product(name, mode):
+    """FP32-mixed matmul: the FP32 tile sums are folded into FP64 every 16
+    inner entries whatever the tile variant, so the variants agree bitwise
+    here too."""
+    h2, _ = problem(name)
+    X = np.random.default_rng(4).standard_normal((h2.krnl_mat_size, 32))
+    m = H2MatHIP(h2, mode=mode, prec="fp32mix", host_blocks=False)
+    base = None
+    for v in range(abi.MM_VARIANTS):
+        m.mm_variant(-1, 32, v)
+        Y = m.matmul(X, "col")
+        if base is None:
+            base = Y
+        assert np.array_equal(Y, base), f"variant {v} differs"
+    m.autotune(32)
+    assert np.array_equal(m.matmul(X, "col"), base)
