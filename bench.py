@@ -10,12 +10,17 @@ product (y = A_H2 x + 1e-2 x).
 
 A step is one (A_H2 + sigma^2 I) x on vectors already resident in HBM.
 `value` is FP64 JIT-mode matvecs per second.  The same run also reports the
-AOT mode (stored blocks, HBM GB/s against the H2Pack byte model), the
-relative error against an FP64 direct n-body product on 1000 sampled rows
-and against the oracle, the roofline of the dominant kernel, and the
-oracle's CPU time on this host.
+FP32-mixed JIT leg, the AOT mode (stored blocks, HBM GB/s against the H2Pack
+byte model), the relative error against an FP64 direct n-body product on
+1000 sampled rows and against the oracle, the roofline of the dominant
+kernel, a serving leg (Matern-3/2 GP requests with Poisson arrivals:
+continuous batching on the FP32-mixed path against one request at a time on
+the FP64 path, p50/p99 latency and requests/s; the requests' H2 matrices are
+built in worker processes while the N = 1M legs run), and the oracle's CPU
+time on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-aot]
+                    [--serve-requests R] [--serve-rate Q]
 """
 from __future__ import annotations
 
@@ -75,6 +80,9 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: most requests on the GPU at once")
     ap.add_argument("--cg-max-iter", type=int, default=3000, help="serving: CG iteration cap")
     ap.add_argument("--build-workers", type=int, default=4, help="serving: host construction processes")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="gaussian1m: requests of the serving leg (0: no serving leg)")
+    ap.add_argument("--serve-rate", type=float, default=1.5, help="gaussian1m: serving leg arrivals per second")
     ap.add_argument("--periodic-n", type=int, default=100_000, help="periodic: particles")
     ap.add_argument("--periodic-rows", type=int, default=10, help="periodic: dense Ewald check rows")
     ap.add_argument("--copy-calib", action="store_true",
@@ -111,6 +119,14 @@ def main():
     t_build = time.perf_counter() - t0
     log(f"[bench] built H2 for N={N} in {t_build:.1f}s: levels={h2.max_level + 1} "
         f"leaves={h2.n_leaf_node} adm={h2.n_r_adm_pair} inadm={h2.n_r_inadm_pair}")
+    # the serving leg's requests are built in worker processes while the
+    # N = 1M legs run on the GPU (started before this process uses the GPU)
+    serve_builds = None
+    if args.serve_requests > 0:
+        import bench_workloads
+        t_sb = time.perf_counter()
+        serve_builds = bench_workloads.start_request_builds(args.serve_requests, args.seed, args.serve_rate,
+                                                            args.cg_max_iter, args.build_workers)
 
     rng = np.random.default_rng(args.seed + 1)
     x = rng.random(N) - 0.5
@@ -230,6 +246,26 @@ def main():
             aot = {"error": str(e)}
             log(f"[bench] AOT leg failed: {e}")
 
+    # ---- serving: Matern-3/2 GP requests, continuous batching (FP32-mixed)
+    # against one request at a time on the FP64 path
+    serving = None
+    if serve_builds is not None:
+        try:
+            reqs = bench_workloads.collect_requests(*serve_builds)
+            t_sb = time.perf_counter() - t_sb
+            r = bench_workloads.serve_compare(reqs, slots=args.serve_requests)
+            d32, d64 = r["batched_fp32mix"], r["alone_fp64"]
+            serving = {"workload": f"serving_matern32_{args.serve_requests}req_N1e4-4e5_density10_reltol1e-6_"
+                                   f"leaf300_poisson{args.serve_rate}_8rhs_cg_tol1e-6_nugget1e-2",
+                       "requests": args.serve_requests, "rate_per_s": args.serve_rate,
+                       "p50_s": d32["p50"], "p99_s": d32["p99"], "throughput_req_per_s": d32["throughput"],
+                       "alone_fp64_p50_s": d64["p50"], "alone_fp64_p99_s": d64["p99"],
+                       "alone_fp64_throughput_req_per_s": d64["throughput"],
+                       "host_build_s_overlapped": t_sb, **r}
+        except Exception as e:   # report, do not hide the headline
+            serving = {"error": str(e)}
+            log(f"[bench] serving leg failed: {e}")
+
     # ---- CPU baseline: the oracle on this host, also the FP64 parity check
     cpu = None
     err_oracle = None
@@ -280,7 +316,8 @@ def main():
         "config": {"workload": f"gaussian3d_l1_N{N}_density10_reltol1e-6_leaf300_jit_fp64_nugget1e-2",
                    "N": N, "mode": "jit", "precision": "fp64", "kernel": "gaussian", "reltol": 1e-6,
                    "leaf": 300, "parallelism": "single-gpu"},
-        "roofline": {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "pipe": "FP64 (VALU and MFMA issue to one shared FP64 pipe)",
+                     "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / FP64_PEAK_TFLOPS,
                      "kernel": ("k_blocks_jit2_occ<1, double, 4, 0>" if jv == 0
                                 else "k_blocks_jit2<1, double, 0>"), "kernel_ms": info_j["t_blocks_b"],
@@ -293,10 +330,12 @@ def main():
                      "traffic_source": "profiles/gaussian_1M_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "note": "achieved = H2Pack JIT_flops model (14 flops per Gaussian pair) / kernel time; "
                              "FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak (one FP64 pipe: "
-                             "SQ_VALU_MFMA_COEXEC_CYCLES = 0); evaluating a pair issues ~20 pipe slots "
-                             "(2^x alone 14 FP64 operations), and rocprofv3 counters put the pipe ~90% busy "
-                             "(profiles/gaussian_1M_jit_pipe_counters.md)"},
+                             "SQ_VALU_MFMA_COEXEC_CYCLES = 0); evaluating a pair issues ~18 pipe slots "
+                             "(2^x 12 FP64/integer operations with its 128-entry LDS table, the squared "
+                             "distance 4 slots of MFMA, the two products 2), and rocprofv3 counters put the "
+                             "pipe ~90% busy (profiles/gaussian_1M_jit_pipe_counters.md)"},
         "cpu_baseline": cpu,
+        "serving": serving,
         "rel_err_vs_direct_nbody_1000rows": err_direct,
         "rel_err_vs_oracle": err_oracle,
         "fp32mix": fp32,

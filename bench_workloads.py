@@ -60,40 +60,54 @@ def _build_worker(spec, kw, threads):
         return r, time.perf_counter() - t
 
 
-def run_serving(args):
-    """Matern-3/2 GP requests through the continuous-batching loop (FP32-mixed
-    JIT, co-scheduled launches) and, on the same arrivals, one request at a
-    time on the plain FP64 path.  Host construction happens before the clock
-    starts, in worker processes that never touch the GPU."""
+def start_request_builds(n, seed, rate, cg_max_iter, workers):
+    """Build the serving requests' H2 matrices in worker processes (which
+    never touch the GPU); returns (executor, futures).  Call this before
+    the GPU is first used: the workers are started by this call."""
     import multiprocessing as mp
     from concurrent.futures import ProcessPoolExecutor
-    from h2pack_amd.serve import Server, request_specs
-    specs = request_specs(args.requests, args.seed, 10_000, 400_000, args.rate)
-    kw = {"max_iter": args.cg_max_iter}
-    workers = max(1, args.build_workers)
+    from h2pack_amd.serve import request_specs
+    specs = request_specs(n, seed, 10_000, 400_000, rate)
+    kw = {"max_iter": cg_max_iter}
+    workers = max(1, workers)
     threads = max(1, 16 // workers)
     # the workers inherit these at start: bounded BLAS / OpenMP threads that
     # sleep instead of spinning when idle, so the processes do not starve
-    # each other
-    os.environ.update({"OMP_WAIT_POLICY": "PASSIVE", "OPENBLAS_NUM_THREADS": str(threads),
-                       "H2P_BUILD_THREADS": str(threads)})
-    t0 = time.perf_counter()
-    reqs = [None] * len(specs)
-    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
-        futs = {ex.submit(_build_worker, s, kw, threads): s["rid"] for s in specs}
-        for f in futs:
-            r, tb = f.result()
-            reqs[futs[f]] = r
-            log(f"[bench] request {r.rid}: {r.kind} N={r.N} built in {tb:.1f}s")
-    t_build = time.perf_counter() - t0
-    for k in ("OMP_WAIT_POLICY", "OPENBLAS_NUM_THREADS", "H2P_BUILD_THREADS"):
-        os.environ.pop(k, None)
+    # each other (restored in this process once they have started)
+    keys = {"OMP_WAIT_POLICY": "PASSIVE", "OPENBLAS_NUM_THREADS": str(threads), "H2P_BUILD_THREADS": str(threads)}
+    saved = {k: os.environ.get(k) for k in keys}
+    os.environ.update(keys)
+    ex = ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
+    futs = [(s["rid"], ex.submit(_build_worker, s, kw, threads)) for s in specs]
+    for k, v in saved.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+    return ex, futs
+
+
+def collect_requests(ex, futs):
+    reqs = [None] * len(futs)
+    for rid, f in futs:
+        r, tb = f.result()
+        reqs[rid] = r
+        log(f"[bench] request {r.rid}: {r.kind} N={r.N} built in {tb:.1f}s")
+    ex.shutdown()
+    return reqs
+
+
+def serve_compare(reqs, slots, hbm_frac=0.8):
+    """The requests through the continuous-batching loop (FP32-mixed JIT,
+    co-scheduled launches) and, on the same arrivals, one at a time on the
+    plain FP64 path; returns the summary dict."""
+    import copy
     import torch
+    from h2pack_amd.serve import Server
     free, total = torch.cuda.mem_get_info()
-    budget = int(0.8 * free)
+    budget = int(hbm_frac * free)
 
     def fresh():
-        import copy
         out = []
         for r in reqs:
             c = copy.copy(r)
@@ -102,10 +116,10 @@ def run_serving(args):
         return out
 
     res = {}
-    for name, prec, slots in (("batched_fp32mix", "fp32mix", args.slots), ("alone_fp64", "fp64", 1)):
+    for name, prec, sl in (("batched_fp32mix", "fp32mix", slots), ("alone_fp64", "fp64", 1)):
         rs = fresh()
-        srv = Server(hbm_budget=budget, max_active=slots, prec=prec, cg_chunk=8)
-        log(f"[bench] serving {len(rs)} requests: {name} ({prec}, up to {slots} at once)")
+        srv = Server(hbm_budget=budget, max_active=sl, prec=prec, cg_chunk=8)
+        log(f"[bench] serving {len(rs)} requests: {name} ({prec}, up to {sl} at once)")
         st = srv.run(rs, realtime=True, log=log)
         srv.close()
         cg = [r for r in rs if r.kind == "cg"]
@@ -116,26 +130,39 @@ def run_serving(args):
         res[name] = (d, rs)
         log(f"[bench] {name}: {st.n_done} done in {st.makespan:.1f}s, {st.throughput:.3f} req/s, "
             f"p50 {st.p50:.2f}s p99 {st.p99:.2f}s, max active {st.max_active}")
-    # the FP32-mixed results against the FP64 ones
+    # the FP32-mixed results against the FP64 ones (the matmul requests)
     errs = []
     for a, b in zip(res["batched_fp32mix"][1], res["alone_fp64"][1]):
         if a.kind == "matmul":
             errs.append(_rel(a.result, b.result))
     d32, d64 = res["batched_fp32mix"][0], res["alone_fp64"][0]
+    return {"batched_fp32mix": d32, "alone_fp64": d64, "hbm_budget_bytes": budget,
+            "throughput_speedup": d32["throughput"] / d64["throughput"] if d64["throughput"] else None,
+            "p50_speedup": d64["p50"] / d32["p50"] if d32["p50"] else None,
+            "p99_speedup": d64["p99"] / d32["p99"] if d32["p99"] else None,
+            "matmul_max_rel_diff_fp32mix_vs_fp64": max(errs) if errs else None,
+            "matmul_min_rel_diff_fp32mix_vs_fp64": min(errs) if errs else None,
+            "mean_N": float(np.mean([r.N for r in reqs]))}
+
+
+def run_serving(args):
+    """Matern-3/2 GP requests: continuous batching on the FP32-mixed path
+    against serving each alone on the FP64 path.  Host construction happens
+    before the clock starts, in worker processes."""
+    t0 = time.perf_counter()
+    ex, futs = start_request_builds(args.requests, args.seed, args.rate, args.cg_max_iter, args.build_workers)
+    reqs = collect_requests(ex, futs)
+    t_build = time.perf_counter() - t0
+    r = serve_compare(reqs, args.slots)
+    d32 = r["batched_fp32mix"]
     out = {"metric": "served GP requests/s and p50/p99 latency (Matern-3/2, FP32-mixed batched vs FP64 alone)",
            "value": d32["throughput"], "unit": "requests/s", "higher_is_better": True,
            "n_gpus": 1, "dtype": "f32-mixed", "data": "synthetic",
            "config": {"workload": f"serving_matern32_{args.requests}req_N1e4-4e5_density10_reltol1e-6_leaf300"
                                   f"_rate{args.rate}_8rhs_cg_tol1e-6_nugget1e-2",
                       "requests": args.requests, "rate_per_s": args.rate, "slots": args.slots,
-                      "hbm_budget_bytes": budget, "cg_max_iter": args.cg_max_iter},
-           "batched_fp32mix": d32, "alone_fp64": d64,
-           "throughput_speedup": d32["throughput"] / d64["throughput"] if d64["throughput"] else None,
-           "p50_speedup": d64["p50"] / d32["p50"] if d32["p50"] else None,
-           "p99_speedup": d64["p99"] / d32["p99"] if d32["p99"] else None,
-           "matmul_max_rel_diff_fp32mix_vs_fp64": max(errs) if errs else None,
-           "host_build_s": t_build, "mean_N": float(np.mean([r.N for r in reqs])),
-           "vs_baseline": None}
+                      "hbm_budget_bytes": r.pop("hbm_budget_bytes"), "cg_max_iter": args.cg_max_iter},
+           **r, "host_build_s": t_build, "vs_baseline": None}
     print(json.dumps(out), flush=True)
 
 

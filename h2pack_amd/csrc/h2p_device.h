@@ -137,10 +137,11 @@ __device__ __forceinline__ double rsqrt_refined(double q) {
 
 // Scalar kernels on prepared squared distances.  Coordinates are scaled
 // before the distance is formed so that the kernels need no parameter:
-//   Gaussian exp(-l r^2)          : scale sqrt(l log2 e)   -> K = 2^(-q)
-//   Matern32 (1+t) e^{-t}, t=sqrt3 l r : scale sqrt3 l     -> K = (1+s) e^(-s), s = sqrt(q)
-//   Coulomb 1/r                   : scale 1                -> K = q == 0 ? diag : rsqrt(q)
-// e2t is the exp2_neg table (LDS), unused by Coulomb.
+//   Gaussian exp(-l r^2)          : scale sqrt(128 l log2 e) -> K = 2^(-q / 128)
+//   Matern32 (1+t) e^{-t}, t=sqrt3 l r : scale sqrt3 l       -> K = (1+s) e^(-s), s = sqrt(q)
+//   Coulomb 1/r                   : scale 1                  -> K = q == 0 ? diag : rsqrt(q)
+// (kscale(kid, p0, true)).  e2t is the exp2_neg128 table (LDS), unused by
+// Coulomb.
 template <int KID>
 __device__ __forceinline__ double kprep(double q, double diag, const double *e2t = nullptr) {
     if constexpr (KID == KC) {
@@ -148,17 +149,17 @@ __device__ __forceinline__ double kprep(double q, double diag, const double *e2t
         const double y = rsqrt_refined(z ? 1.0 : q);   // branch-free
         return z ? diag : y;
     } else if constexpr (KID == KG) {
-        return exp2_neg(q, e2t);
+        return exp2_neg128(q, e2t);
     } else if constexpr (KID == KGW) {
-        return exp2_neg_clamped(q, e2t);
+        return exp2_neg128_clamped(q, e2t);
     } else {
         static_assert(KID == KM || KID == KMW, "scalar kernels only");
         // sqrt(q) as q / sqrt(q); the clamp keeps q = 0 (and a rounded
         // negative q from the norm-expanded distance) finite, s ~ 1e-150
         const double qq = __builtin_fmax(q, 1e-300);
         const double s = qq * rsqrt_refined(qq);
-        constexpr double c = 128.0 * 1.4426950408889634;   // e^(-s) = 2^(-s c / 128)
-        const double e = KID == KM ? exp2_neg_c(s, c, e2t) : exp2_neg_c_clamped(s, c, e2t);
+        const double x = s * (128.0 * 1.4426950408889634);   // e^(-s) = 2^(-x / 128)
+        const double e = KID == KM ? exp2_neg128(x, e2t) : exp2_neg128_clamped(x, e2t);
         return __builtin_fma(s, e, e);
     }
 }
@@ -176,9 +177,10 @@ __device__ __forceinline__ float kprep(float q, float diag) {
     }
 }
 
-// Coordinate scale of the prepared form above.
-__host__ __device__ inline double kscale(int kid, double p0) {
-    if (kbase(kid) == KG) return __builtin_sqrt(p0 * 1.4426950408889634);
+// Coordinate scale of the prepared forms: the FP64 kernels above (f64) and
+// the FP32 ones (Gaussian K = 2^(-q) there).
+__host__ __device__ inline double kscale(int kid, double p0, bool f64) {
+    if (kbase(kid) == KG) return __builtin_sqrt(p0 * 1.4426950408889634 * (f64 ? 128.0 : 1.0));
     if (kbase(kid) == KM) return 1.7320508075688772 * p0;
     return 1.0;
 }

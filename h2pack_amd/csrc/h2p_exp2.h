@@ -1,31 +1,31 @@
-// Base-2 exponential 2^(-q) for q >= 0 in FP64, table-driven (Tang's
+// Base-2 exponential 2^(-x / 128) for x >= 0 in FP64, table-driven (Tang's
 // method: P. T. P. Tang, "Table-driven implementation of the exponential
 // function in IEEE floating-point arithmetic", ACM TOMS 15(2), 1989, here in
 // base 2 with 128 table entries).  Used by the FP64 Gaussian and Matern-3/2
-// kernel evaluations of the JIT block launches; the reference evaluates the
-// same kernels through its SIMD library's exp (src/H2Pack_3D_kernels.h:
-// 212-268, 433-497).
+// kernel evaluations of the JIT block launches, whose arguments arrive
+// already multiplied by 128 (the Gaussian through its coordinate scale);
+// the reference evaluates the same kernels through its SIMD library's exp
+// (src/H2Pack_3D_kernels.h:212-268, 433-497).
 //
-// With k = round(-128 q) and u = -128 q - k in [-1/2, 1/2] (both exact):
-//     2^(-q) = 2^floor(k / 128) * 2^((k mod 128) / 128) * 2^(u / 128).
-// k comes from the magic-number rounding t = fma(q, -128, 1.5 2^52): t's
-// last place is 1, so its 52-bit fraction field holds 2^51 + k, whose low
-// 7 bits index the table and whose bits 7..38 are floor(k / 128) as a
-// 32-bit integer.  2^(u/128) = 1 + u P(u) with P of degree 3, minimax for
-// the relative error over u in [-1/2, 1/2] (truncation error 7.6e-17); the
-// table holds 2^(j/128) rounded to nearest.  Max relative error ~2.3e-16
-// (checked against long double exp2 on the host).
+// With k = round(-x) and u = -x - k in [-1/2, 1/2] (both exact):
+//     2^(-x/128) = 2^floor(k / 128) * 2^((k mod 128) / 128) * 2^(u / 128).
+// k comes from the magic-number rounding t = 1.5 2^52 - x: t's last place is
+// 1, so its 52-bit fraction field holds 2^51 + k, whose low 7 bits index
+// the table and whose bits 7..38 are floor(k / 128) as a 32-bit integer.
+// 2^(u/128) = 1 + u P(u) with P of degree 3, minimax for the relative error
+// over u in [-1/2, 1/2] (truncation error 7.6e-17); the table holds
+// 2^(j/128) rounded to nearest.  Max relative error ~2.3e-16 (checked
+// against long double exp2 on the host).
 //
-// Range: exact reduction and a correct exponent need 128 q < 2^51 and
-// |floor(k / 128)| < 2^31, i.e. q < 2^31.  Callers that cannot bound q
-// clamp it first (exp2_neg_clamped): 2^(-q) underflows to 0 well before.
-// exp2_neg_c(x, c) is 2^(-x c / 128), which lets the Matern kernel fold
-// its factor log2(e) into the reduction.
+// Range: exact reduction and a correct exponent need x < 2^38.  Callers
+// that cannot bound x clamp it first (exp2_neg128_clamped): the result
+// underflows to 0 well before.
 //
-// Per value: 3 FMA/adds for the reduction, 2 integer operations for the
-// table address, 1 bit-field extraction for the exponent, 5 FMA/multiplies
-// for the polynomial and table factor, 1 ldexp: 12 VALU instructions
-// (the former degree-10 Chebyshev form took 14).
+// Per value: 3 adds for the reduction (each with at most one scalar
+// operand), 2 integer operations for the table address, 1 bit-field
+// extraction for the exponent, 5 FMA/multiplies for the polynomial and the
+// table factor, 1 ldexp: 12 VALU instructions (the former degree-10
+// Chebyshev form on [-1/2, 1/2] took 14).
 #pragma once
 #include <stdint.h>
 
@@ -72,14 +72,12 @@ namespace h2p {
 
 #define H2P_E2_N 128
 
-// 2^(-x c / 128) for x c < 2^38 (c > 0; see above); tab = H2P_E2_TABLE.
-// With c = 128 the reduction is exact; with another c the product x c is
-// exact inside the two FMAs and u is rounded once.
-H2P_HD inline double exp2_neg_c(double x, double c, const double *tab) {
+// 2^(-x / 128) for 0 <= x < 2^38 (see above); tab = H2P_E2_TABLE.
+H2P_HD inline double exp2_neg128(double x, const double *tab) {
     const double magic = 0x1.8p52;
-    const double t = __builtin_fma(x, -c, magic);
-    const double kd = t - magic;                   // k, exact
-    const double u = __builtin_fma(x, -c, -kd);    // -x c - k
+    const double t = magic - x;
+    const double kd = t - magic;                   // k = round(-x), exact
+    const double u = -x - kd;                      // exact
     uint64_t bits;
     __builtin_memcpy(&bits, &t, sizeof bits);
     const double T = tab[(uint32_t)bits & (H2P_E2_N - 1)];
@@ -90,16 +88,10 @@ H2P_HD inline double exp2_neg_c(double x, double c, const double *tab) {
     return __builtin_ldexp(__builtin_fma(T * u, p, T), m);
 }
 
-// 2^(-q) for 0 <= q < 2^31.
-H2P_HD inline double exp2_neg(double q, const double *tab) { return exp2_neg_c(q, 128.0, tab); }
-
-// The same for any x >= 0 (or +inf) and c >= 128: x is clamped to 1100
-// first, where the result is already 0.
-H2P_HD inline double exp2_neg_c_clamped(double x, double c, const double *tab) {
-    return exp2_neg_c(__builtin_fmin(x, 1100.0), c, tab);
-}
-H2P_HD inline double exp2_neg_clamped(double q, const double *tab) {
-    return exp2_neg_c_clamped(q, 128.0, tab);
+// The same for any x >= 0 (or +inf): x is clamped to 128 * 1100 first,
+// where the result is already 0.
+H2P_HD inline double exp2_neg128_clamped(double x, const double *tab) {
+    return exp2_neg128(__builtin_fmin(x, 128.0 * 1100.0), tab);
 }
 
 }  // namespace h2p
