@@ -157,6 +157,16 @@ struct h2p_hip_handle {
     int use_graph = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // the whole matvec is one graph: the gather of the caller's x, the
+    // interior and the leaf outputs into the caller's y; the two boundary
+    // kernel nodes get the next call's x / y when they change
+    hipGraphNode_t gnode_gather = nullptr, gnode_leaf = nullptr;
+    const double *graph_x = nullptr;
+    double *graph_y = nullptr;
+    // recorded after every graph replay of this handle (matvec and matmul),
+    // so that dropping a graph waits for this handle's work only
+    hipEvent_t ev_replay = nullptr;
+    bool replay_pending = false;
     // the interior of a matmul chunk (all product phases on the panels), one
     // graph per panel width; dropped when the panels move or a variant changes
     struct MMGraph { int nvp; hipGraph_t g; hipGraphExec_t e; };
@@ -191,6 +201,7 @@ static void free_handle(h2p_hip_handle *h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_mid) (void)hipEventDestroy(h->ev_mid);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->ev_replay) (void)hipEventDestroy(h->ev_replay);
     delete h;
 }
 
@@ -218,15 +229,15 @@ static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s, const h2p_hip_han
     const BlkTask *tasks = h->d_blk + P.off;
     if constexpr (KID == KGW || KID == KMW) {   // one build for the clamped kernels
         hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, 4, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
-                           h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+                           h->d_coord, h->CT, kscale(KID, h->kp.p0, sizeof(T) == 8), h->kp.p0, h->d_xin, h->d_arena);
         return;
     }
     if (h->mv_var == 1)
         hipLaunchKernelGGL((k_blocks_jit2<KID, T, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
-                           h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+                           h->d_coord, h->CT, kscale(KID, h->kp.p0, sizeof(T) == 8), h->kp.p0, h->d_xin, h->d_arena);
     else
         hipLaunchKernelGGL((k_blocks_jit2_occ<KID, T, 4, PART>), dim3(P.units), dim3(256), 0, s, tasks, P.big, P.n,
-                           h->d_coord, h->CT, kscale(KID, h->kp.p0), h->kp.p0, h->d_xin, h->d_arena);
+                           h->d_coord, h->CT, kscale(KID, h->kp.p0, sizeof(T) == 8), h->kp.p0, h->d_xin, h->d_arena);
 }
 
 template <int KID, typename T>
@@ -343,52 +354,144 @@ static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
     if (fork) (void)hipStreamWaitEvent(s, h->ev_join, 0);
 }
 
-static int ensure_graph(h2p_hip_handle *h) {
-    if (h->graph_exec) return 0;
-    hipStream_t cs = h->own_stream;
-    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    launch_interior(h, cs, nullptr);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(cs, &g);
-    if (e != hipSuccess) return fail(H2P_HIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    h->graph = g;
-    HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
-    return 0;
-}
-
-static void drop_matvec_graph(h2p_hip_handle *h) {
-    if (!h->graph_exec) return;
-    (void)hipDeviceSynchronize();   // a replay may still be running
-    (void)hipGraphExecDestroy(h->graph_exec);
-    (void)hipGraphDestroy(h->graph);
-    h->graph_exec = nullptr;
-    h->graph = nullptr;
-}
-
-// One matvec on device pointers (x, y in the caller's order).
-static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
-    const bool prof = h->profiling && h->ev_used < h->ev_cap;
-    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * H2P_EV_PER_CALL] : nullptr;
-    if (prof) HIP_TRY(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
-    if (!prof && h->use_graph) {
-        int rc = ensure_graph(h);
-        if (rc) return rc;
-        HIP_TRY(hipGraphLaunch(h->graph_exec, s));
-    } else {
-        launch_interior(h, s, ev);
-    }
+// The leaf outputs of a matvec into the caller's y (the last launch).
+static void launch_leaf_out(h2p_hip_handle *h, double *d_y, hipStream_t s) {
     if (h->split)
         hipLaunchKernelGGL(k_leaf_d, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
                            h->d_yt, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
     else
         hipLaunchKernelGGL(k_leaf_final2, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena,
                            h->d_UT, h->d_yp, h->d_fwd, d_y, (const double *)h->d_xin, 0.0);
+}
+
+// Point the graph's gather node at x and its leaf-output node at y.  The
+// argument lists repeat the kernels' parameters one for one.
+static int graph_set_xy(h2p_hip_handle *h, const double *d_x, double *d_y) {
+    if (d_x != h->graph_x) {
+        hipKernelNodeParams p{};
+        HIP_TRY(hipGraphKernelNodeGetParams(h->gnode_gather, &p));
+        const double *a0 = d_x;
+        const int *a1 = h->d_fwd;
+        double *a2 = h->d_xin;
+        int a3 = h->N;
+        void *args[] = {(void *)&a0, (void *)&a1, (void *)&a2, (void *)&a3};
+        p.kernelParams = args;
+        p.extra = nullptr;
+        HIP_TRY(hipGraphExecKernelNodeSetParams(h->graph_exec, h->gnode_gather, &p));
+        h->graph_x = d_x;
+    }
+    if (d_y != h->graph_y) {
+        hipKernelNodeParams p{};
+        HIP_TRY(hipGraphKernelNodeGetParams(h->gnode_leaf, &p));
+        const LeafTask2 *a0 = h->d_leaf;
+        const long long *a1 = h->d_contrib;
+        const double *a2 = h->d_arena;
+        const double *a3 = h->split ? (const double *)h->d_yt : (const double *)h->d_UT;
+        const double *a4 = h->d_yp;
+        const int *a5 = h->d_fwd;
+        double *a6 = d_y;
+        const double *a7 = h->d_xin;
+        double a8 = 0.0;
+        void *split_args[] = {(void *)&a0, (void *)&a1, (void *)&a2, (void *)&a3, (void *)&a5, (void *)&a6,
+                              (void *)&a7, (void *)&a8};
+        void *final_args[] = {(void *)&a0, (void *)&a1, (void *)&a2, (void *)&a3, (void *)&a4, (void *)&a5,
+                              (void *)&a6, (void *)&a7, (void *)&a8};
+        p.kernelParams = h->split ? split_args : final_args;
+        p.extra = nullptr;
+        HIP_TRY(hipGraphExecKernelNodeSetParams(h->graph_exec, h->gnode_leaf, &p));
+        h->graph_y = d_y;
+    }
+    return 0;
+}
+
+// Capture gather + interior + leaf outputs once (for x, y) and find the two
+// boundary kernel nodes.
+static int ensure_graph(h2p_hip_handle *h, const double *d_x, double *d_y) {
+    if (h->graph_exec) return graph_set_xy(h, d_x, d_y);
+    hipStream_t cs = h->own_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, cs, d_x, h->d_fwd, h->d_xin, h->N);
+    launch_interior(h, cs, nullptr);
+    launch_leaf_out(h, d_y, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    if (e != hipSuccess) return fail(H2P_HIP_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    size_t n = 0;
+    HIP_TRY(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    HIP_TRY(hipGraphGetNodes(g, nodes.data(), &n));
+    const void *f_gather = reinterpret_cast<const void *>(&k_gather);
+    const void *f_leaf = h->split ? reinterpret_cast<const void *>(&k_leaf_d)
+                                  : reinterpret_cast<const void *>(&k_leaf_final2);
+    hipGraphNode_t ng = nullptr, nl = nullptr;
+    int n_gather = 0, n_leaf = 0;
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType ty;
+        HIP_TRY(hipGraphNodeGetType(nd, &ty));
+        if (ty != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams p{};
+        HIP_TRY(hipGraphKernelNodeGetParams(nd, &p));
+        if (p.func == f_gather) { ng = nd; n_gather++; }
+        if (p.func == f_leaf) { nl = nd; n_leaf++; }
+    }
+    if (n_gather != 1 || n_leaf != 1) {
+        (void)hipGraphDestroy(g);
+        return fail(H2P_HIP_EDEVICE, "captured matvec graph: boundary kernel nodes not found");
+    }
+    h->graph = g;
+    h->gnode_gather = ng;
+    h->gnode_leaf = nl;
+    HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    h->graph_x = d_x;
+    h->graph_y = d_y;
+    return 0;
+}
+
+// Record the end of a call's work on stream s, and wait for the handle's
+// last recorded call (matvec, matmul or batched), if any.  Calls on one
+// handle do not overlap, so the last one covers the earlier ones.
+static void mark_use(h2p_hip_handle *h, hipStream_t s) {
+    if (hipEventRecord(h->ev_replay, s) == hipSuccess) h->replay_pending = true;
+}
+
+static void wait_replays(h2p_hip_handle *h) {
+    if (h->replay_pending) (void)hipEventSynchronize(h->ev_replay);
+    h->replay_pending = false;
+}
+
+static void drop_matvec_graph(h2p_hip_handle *h) {
+    if (!h->graph_exec) return;
+    wait_replays(h);
+    (void)hipGraphExecDestroy(h->graph_exec);
+    (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->gnode_gather = h->gnode_leaf = nullptr;
+    h->graph_x = nullptr;
+    h->graph_y = nullptr;
+}
+
+// One matvec on device pointers (x, y in the caller's order).
+static int run_matvec(h2p_hip_handle *h, const double *d_x, double *d_y, hipStream_t s) {
+    const bool prof = h->profiling && h->ev_used < h->ev_cap;
+    if (!prof && h->use_graph) {
+        int rc = ensure_graph(h, d_x, d_y);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(h->graph_exec, s));
+        mark_use(h, s);
+        return 0;
+    }
+    hipEvent_t *ev = prof ? &h->ev_pool[(size_t)h->ev_used * H2P_EV_PER_CALL] : nullptr;
+    if (prof) HIP_TRY(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_gather, dim3((h->N + 255) / 256), dim3(256), 0, s, d_x, h->d_fwd, h->d_xin, h->N);
+    launch_interior(h, s, ev);
+    launch_leaf_out(h, d_y, s);
     HIP_TRY(hipGetLastError());
     if (prof) {
         HIP_TRY(hipEventRecord(ev[6], s));
         h->ev_used++;
     }
+    mark_use(h, s);
     return 0;
 }
 
@@ -493,7 +596,7 @@ static void launch_mm_phase(const h2p_hip_handle *h, int ph, const MMPanels &P, 
 
 static void drop_mm_graphs(h2p_hip_handle *h) {
     if (h->mm_graphs.empty()) return;
-    (void)hipDeviceSynchronize();   // a replay may still be running
+    wait_replays(h);   // a replay may still be running
     for (auto &m : h->mm_graphs) {
         (void)hipGraphExecDestroy(m.e);
         (void)hipGraphDestroy(m.g);
@@ -673,7 +776,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             }
             ext = std::max(ext, hi - lo);
         }
-        const double rmax = 1.7320508075688772 * ext * kscale(krnl->id, h->kp.p0);
+        const double rmax = 1.7320508075688772 * ext * kscale(krnl->id, h->kp.p0, false);
         const double arg = krnl->id == KG ? rmax * rmax : rmax * 1.4426950408889634;
         h->wide = !(arg < 0x1p30);   // also for non-finite coordinates or parameters
     }
@@ -1251,6 +1354,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         }
     }
     if (hipStreamCreate(&h->own_stream) != hipSuccess) return bail(fail(H2P_HIP_EDEVICE, "stream create failed"));
+    if (hipEventCreateWithFlags(&h->ev_replay, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(H2P_HIP_EDEVICE, "event create failed"));
     if (h->split) {
         if (hipStreamCreate(&h->aux_stream) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1299,8 +1404,9 @@ static int ensure_panels(h2p_hip_handle *h, int nvp) {
     const size_t rows = panel_rows(h->N, h->S);
     drop_mm_graphs(h);
     if (h->d_panels) {
-        // earlier products on any stream may still read the old panels
-        HIP_TRY(hipDeviceSynchronize());
+        // the handle's last product (on whichever stream) may still read the
+        // old panels
+        wait_replays(h);
         (void)hipFree(h->d_panels);
         h->d_panels = nullptr;
         h->device_bytes -= sizeof(double) * rows * h->panel_nvp;
@@ -1414,6 +1520,7 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
                            h->d_fwd, N, d_Y, (long long)ldy, rm ? 1 : 0, v0, nv);
         HIP_TRY(hipGetLastError());
     }
+    mark_use(h, s);
     return 0;
 }
 
@@ -1476,18 +1583,36 @@ int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, in
     return rc;
 }
 
-// Mean time of the matvec interior (graph replay, or eager launches with
-// graphs off) over reps calls after one warm-up, on the handle's stream.
+// Mean time of the matvec interior (replays of a graph of it captured for
+// the timing, or eager launches with graphs off) over reps calls after one
+// warm-up, on the handle's stream.
 static int time_interior(h2p_hip_handle *h, int reps, double *ms) {
     hipStream_t s = h->own_stream;
-    auto once = [&]() -> int {
-        if (h->use_graph) {
-            int rc = ensure_graph(h);
-            if (rc) return rc;
-            HIP_TRY(hipGraphLaunch(h->graph_exec, s));
-        } else {
-            launch_interior(h, s, nullptr);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (h->use_graph) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_interior(h, s, nullptr);
+        HIP_TRY(hipStreamEndCapture(s, &g));
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(H2P_HIP_EDEVICE, "hipGraphInstantiate failed");
         }
+    }
+    struct Drop {
+        hipGraph_t g;
+        hipGraphExec_t e;
+        hipStream_t s;
+        ~Drop() {
+            if (!e) return;
+            (void)hipStreamSynchronize(s);
+            (void)hipGraphExecDestroy(e);
+            (void)hipGraphDestroy(g);
+        }
+    } drop{g, ge, s};
+    auto once = [&]() -> int {
+        if (ge) HIP_TRY(hipGraphLaunch(ge, s));
+        else launch_interior(h, s, nullptr);
         return 0;
     };
     int rc = once();
@@ -1840,7 +1965,7 @@ static MVPlan plan_matvec(TableBuf &tb, int n, h2p_hip_handle *const *hs, const 
         ReqCtx &c = reqs[i];
         c.coord = h->d_coord;
         c.CT = h->CT;
-        c.scale = kscale(h->kid, h->kp.p0);
+        c.scale = kscale(h->kid, h->kp.p0, h->prec == H2P_HIP_FP64);
         c.diag = h->kp.p0;
         c.kp = h->kp;
         c.xin = h->d_xin;
@@ -2237,6 +2362,8 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
                                ls.n_items, dctx);
         if ((rc = release_slot(B, s))) return rc;
     }
+    for (int i = 0; i < n; i++)
+        if (n_vec[i] > 0) mark_use(hs[i], s);   // their panels are in use on s
     return 0;
 }
 

@@ -204,6 +204,46 @@ def test_graph_replays_match_eager_path(mode):
         assert np.array_equal(y_graph, y_eager), it
 
 
+@pytest.mark.parametrize("name,mode", [("gaussian", "jit"), ("coulomb", "aot"), ("rpy", "jit")])
+def test_whole_graph_follows_caller_pointers(name, mode):
+    """The whole matvec (gather of x, interior, leaf outputs into y) is one
+    graph; calls with other x / y device addresses patch its two boundary
+    kernel nodes.  Replays alternating between buffers, reusing one, and
+    writing into a buffer that was an input before, each equal the eager
+    (profiled) result bit for bit, and leave the other buffers untouched."""
+    import torch
+    h2, _ = problem(name)
+    m = H2MatHIP(h2, mode=mode, host_blocks=False)
+    N = h2.krnl_mat_size
+    rng = np.random.default_rng(31)
+    xs = [torch.from_numpy(rng.random(N) - 0.5).to("cuda:0") for _ in range(3)]
+    ys = [torch.full((N,), np.nan, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    eager = []
+    m.set_profiling(True)                       # profiled calls launch eagerly
+    for x in xs:
+        y = torch.empty_like(x)
+        m.matvec_dev(x.data_ptr(), y.data_ptr())
+        torch.cuda.synchronize()
+        eager.append(y.cpu().numpy())
+    m.info()
+    m.set_profiling(False)
+    order = [(0, 0), (1, 1), (2, 2), (0, 1), (2, 0), (1, 2), (0, 0)]
+    for i, j in order:
+        before = [y.clone() for y in ys]
+        m.matvec_dev(xs[i].data_ptr(), ys[j].data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(ys[j].cpu().numpy(), eager[i]), (i, j)
+        for k in range(3):
+            if k != j:
+                assert torch.equal(ys[k], before[k]) or (torch.isnan(ys[k]).all() and torch.isnan(before[k]).all())
+    # y into an earlier input buffer: the graph's x and y move independently
+    x_copy = xs[1].clone()
+    m.matvec_dev(x_copy.data_ptr(), xs[0].data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(xs[0].cpu().numpy(), eager[1])
+    m.close()
+
+
 @pytest.mark.parametrize("d_late", [None, "0", "1"])
 @pytest.mark.parametrize("name,prec", [("gaussian", "fp64"), ("coulomb", "fp64"), ("matern32", "fp32mix")])
 def test_split_block_launches_agree(name, prec, d_late, monkeypatch):
