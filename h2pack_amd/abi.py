@@ -15,8 +15,10 @@ EXPORTS = ("h2p_hip_upload", "h2p_hip_matvec", "h2p_hip_matvec_dev", "h2p_hip_ma
            "h2p_hip_free", "h2p_hip_last_error", "h2p_hip_direct_nbody",
            "h2p_hip_footprint", "h2p_hip_batch_create", "h2p_hip_batch_free",
            "h2p_hip_matvec_batch", "h2p_hip_matmul_batch", "h2p_hip_cg_start",
-           "h2p_hip_cg_step", "h2p_hip_cg_poll", "h2p_hip_autotune", "h2p_hip_mm_variant")
+           "h2p_hip_cg_step", "h2p_hip_cg_poll", "h2p_hip_autotune", "h2p_hip_mm_variant",
+           "h2p_hip_matmul_dev_noalloc")
 MM_VARIANTS = 4
+NOALLOC_BYTES = 256 << 20
 
 _vp = ct.c_void_p
 
@@ -76,6 +78,8 @@ def bind(lib: ct.CDLL) -> ct.CDLL:
     lib.h2p_hip_matmul.restype = ct.c_int
     lib.h2p_hip_matmul_dev.argtypes = [_vp, ct.c_int, ct.c_int, _vp, ct.c_int, _vp, ct.c_int, _vp]
     lib.h2p_hip_matmul_dev.restype = ct.c_int
+    lib.h2p_hip_matmul_dev_noalloc.argtypes = [_vp, ct.c_int, ct.c_int, _vp, ct.c_int, _vp, ct.c_int, _vp]
+    lib.h2p_hip_matmul_dev_noalloc.restype = ct.c_int
     lib.h2p_hip_set_profiling.argtypes = [_vp, ct.c_int]
     lib.h2p_hip_set_profiling.restype = ct.c_int
     lib.h2p_hip_get_info.argtypes = [_vp, ct.POINTER(Info)]

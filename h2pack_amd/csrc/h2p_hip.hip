@@ -1420,10 +1420,10 @@ static int ensure_panels(h2p_hip_handle *h, int nvp) {
     return 0;
 }
 
-static MMPanels panels_of(const h2p_hip_handle *h, int nvp) {
+static MMPanels panels_of(const h2p_hip_handle *h, int nvp, double *base) {
     MMPanels P;
     P.nvp = nvp;
-    P.p[MM_PX] = h->d_panels;
+    P.p[MM_PX] = base;
     P.p[MM_Y0] = P.p[MM_PX] + (size_t)h->N * nvp;
     P.p[MM_Y1] = P.p[MM_Y0] + (size_t)h->S * nvp;
     P.p[MM_PY] = P.p[MM_Y1] + (size_t)h->S * nvp;
@@ -1485,9 +1485,12 @@ int h2p_hip_matvec(h2p_hip_handle *h, const double *x, double *y) {
     return 0;
 }
 
-int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d_X, int ldx,
-                       double *d_Y, int ldy, void *stream) {
-    if (!h || (n_vec > 0 && (!d_X || !d_Y))) return fail(H2P_HIP_EINVAL, "NULL argument");
+}  // extern "C"
+
+// Argument checks shared by the matmul entry points.
+static int check_matmul_args(const h2p_hip_handle *h, int layout, int n_vec, const void *X, int ldx, const void *Y,
+                             int ldy) {
+    if (!h || (n_vec > 0 && (!X || !Y))) return fail(H2P_HIP_EINVAL, "NULL argument");
     if (n_vec < 0) return fail(H2P_HIP_EINVAL, "n_vec < 0");
     if (n_vec == 0) return 0;
     const bool rm = layout == H2P_HIP_ROW_MAJOR;
@@ -1495,20 +1498,28 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
     const int N = h->N;
     if (rm ? (ldx < n_vec || ldy < n_vec) : (ldx < N || ldy < N)) return fail(H2P_HIP_EINVAL, "bad leading dimension");
     if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
-    hipStream_t s = (hipStream_t)stream;
-    int rc = ensure_panels(h, panel_width(n_vec));
-    if (rc) return rc;
+    return 0;
+}
+
+// The chunks of one matmul on panels starting at `panels` (panel_rows x
+// panel_width(n_vec) doubles).  graphs: replay the handle's captured chunk
+// graphs; otherwise every phase is launched directly.
+static int run_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *d_X, int ldx, double *d_Y, int ldy,
+                      hipStream_t s, double *panels, bool graphs) {
+    const bool rm = layout == H2P_HIP_ROW_MAJOR;
+    const int N = h->N;
+    int rc = 0;
     // columns in chunks of at most 64 (the reference chunks by mm_max_n_vec,
     // src/H2Pack_matmul.c:497-580)
     for (int v0 = 0; v0 < n_vec; v0 += H2P_MM_MAXNVP) {
         const int nv = std::min(H2P_MM_MAXNVP, n_vec - v0);
         const int nvp = (nv + 15) & ~15;
-        const MMPanels P = panels_of(h, nvp);
+        const MMPanels P = panels_of(h, nvp, panels);
         const long long ne = (long long)N * nvp;
         hipLaunchKernelGGL(k_mm_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, d_X, (long long)ldx,
                            rm ? 1 : 0, v0, nv, h->d_fwd, N, P.p[MM_PX], nvp);
         const int c = nv_class(nvp);
-        if (h->use_graph) {
+        if (graphs) {
             hipGraphExec_t ge = nullptr;
             if ((rc = mm_graph(h, P, &ge))) return rc;
             HIP_TRY(hipGraphLaunch(ge, s));
@@ -1522,6 +1533,57 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d
     }
     mark_use(h, s);
     return 0;
+}
+
+// Panels of the allocation-free matmul: one static device array per GPU,
+// shared by every call on that GPU.  Host code never passes `scratch`
+// itself (that is a host-side stand-in); it passes the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device on its first call.
+#define H2P_SCRATCH_BYTES H2P_HIP_NOALLOC_BYTES
+static __device__ __align__(256) double scratch[H2P_SCRATCH_BYTES / sizeof(double)];
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;
+
+static int scratch_ptr(double **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return 0;
+    }
+    double *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(H2P_HIP_EDEVICE, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = p;
+    *out = p;
+    return 0;
+}
+
+extern "C" {
+
+int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec, const double *d_X, int ldx,
+                       double *d_Y, int ldy, void *stream) {
+    int rc = check_matmul_args(h, layout, n_vec, d_X, ldx, d_Y, ldy);
+    if (rc || n_vec == 0) return rc;
+    if ((rc = ensure_panels(h, panel_width(n_vec)))) return rc;
+    return run_matmul(h, layout, n_vec, d_X, ldx, d_Y, ldy, (hipStream_t)stream, h->d_panels, h->use_graph != 0);
+}
+
+int h2p_hip_matmul_dev_noalloc(h2p_hip_handle *h, int layout, int n_vec, const double *d_X, int ldx,
+                               double *d_Y, int ldy, void *stream) {
+    int rc = check_matmul_args(h, layout, n_vec, d_X, ldx, d_Y, ldy);
+    if (rc || n_vec == 0) return rc;
+    const size_t need = sizeof(double) * panel_rows(h->N, h->S) * panel_width(n_vec);
+    if (need > H2P_SCRATCH_BYTES)
+        return fail(H2P_HIP_EINVAL, "matmul panels need " + std::to_string(need) + " bytes, more than the " +
+                                        std::to_string(H2P_SCRATCH_BYTES) + "-byte static scratch");
+    double *panels = nullptr;
+    if ((rc = scratch_ptr(&panels))) return rc;
+    // no graph: instantiating one allocates
+    return run_matmul(h, layout, n_vec, d_X, ldx, d_Y, ldy, (hipStream_t)stream, panels, false);
 }
 
 // Tile-variant choices shared by handles whose matmul phase has the same
@@ -1719,7 +1781,7 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
     const int nvp = panel_width(n_vec), cls = nv_class(nvp);
     int rc = ensure_panels(h, nvp);
     if (rc) return rc;
-    const MMPanels P = panels_of(h, nvp);
+    const MMPanels P = panels_of(h, nvp, h->d_panels);
     hipStream_t s = h->own_stream;
     HIP_TRY(hipMemsetAsync(h->d_panels, 0, sizeof(double) * panel_rows(h->N, h->S) * nvp, s));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2263,7 +2325,7 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
             c.coord = h->d_coord;
             c.CT = h->CT;
             c.kp = h->kp;
-            c.P = panels_of(h, (c.nv + 15) & ~15);
+            c.P = panels_of(h, (c.nv + 15) & ~15, h->d_panels);
             c.fwd = h->d_fwd;
             c.N = h->N;
             c.X = d_X[i];

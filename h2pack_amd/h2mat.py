@@ -161,6 +161,13 @@ class H2MatHIP:
         _check(self.lib.h2p_hip_matmul_dev(self.handle, layout, n_vec, ct.c_void_p(x_ptr), ldx,
                                            ct.c_void_p(y_ptr), ldy, ct.c_void_p(stream)))
 
+    def matmul_dev_noalloc(self, layout: int, n_vec: int, x_ptr: int, ldx: int, y_ptr: int, ldy: int,
+                           stream: int = 0):
+        """matmul_dev with its panels in the library's static per-GPU scratch
+        (h2p_hip_matmul_dev_noalloc): no device allocation in the call."""
+        _check(self.lib.h2p_hip_matmul_dev_noalloc(self.handle, layout, n_vec, ct.c_void_p(x_ptr), ldx,
+                                                   ct.c_void_p(y_ptr), ldy, ct.c_void_p(stream)))
+
     def autotune(self, n_vec: int) -> dict:
         """Time the kernel variants on this matrix and keep the fastest
         (h2p_hip_autotune): the matmul tiles per phase for ``n_vec`` columns,

@@ -165,6 +165,22 @@ int h2p_hip_matmul_dev(h2p_hip_handle *h, int layout, int n_vec,
                        const double *d_X, int ldx, double *d_Y, int ldy,
                        void *stream);
 
+/* h2p_hip_matmul_dev without any allocation.  No counterpart in the
+ * reference: this is for callers that cannot allocate device memory during
+ * the call (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same kernels and tile variants as
+ * h2p_hip_matmul_dev, bitwise the same result, launched directly (no
+ * graph); the panels (permuted X, skeleton coefficients, permuted Y) live
+ * in one static device array of H2P_HIP_NOALLOC_BYTES per GPU instead of
+ * the handle's lazily allocated panels.  The array is shared by every call
+ * on that GPU, so calls that overlap on one GPU (any handle, any stream)
+ * race; it needs (2 krnl_mat_size + 2 sum(U_ncol)) * min(64, n_vec rounded
+ * up to 16) doubles, and larger requests return H2P_HIP_EINVAL. */
+#define H2P_HIP_NOALLOC_BYTES ((size_t)256 << 20)
+int h2p_hip_matmul_dev_noalloc(h2p_hip_handle *h, int layout, int n_vec,
+                               const double *d_X, int ldx, double *d_Y,
+                               int ldy, void *stream);
+
 /* Autotuner of the matmul tiles.  Each matmul phase (upward sweep, coupling,
  * downward sweep, leaf outputs) runs as grouped MFMA launches whose tile
  * geometry (rows per chunk, inner entries per LDS stage) is one of
