@@ -316,7 +316,10 @@ def main():
         "config": {"workload": f"gaussian3d_l1_N{N}_density10_reltol1e-6_leaf300_jit_fp64_nugget1e-2",
                    "N": N, "mode": "jit", "precision": "fp64", "kernel": "gaussian", "reltol": 1e-6,
                    "leaf": 300, "parallelism": "single-gpu"},
-        "roofline": {"bound": "mfma", "pipe": "FP64 (VALU and MFMA issue to one shared FP64 pipe)",
+        "roofline": {"bound": "mfma",
+                     "pipe": "FP64 issue: the launch is compute bound on the shared FP64 pipe (VALU kernel "
+                             "evaluation plus the MFMA distance tiles), not on HBM; 'mfma' names the compute "
+                             "bound, the peak is the FP64 one",
                      "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / FP64_PEAK_TFLOPS,
                      "kernel": ("k_blocks_jit2_occ<1, double, 4, 0>" if jv == 0
