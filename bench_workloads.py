@@ -221,6 +221,8 @@ def run(args):
         tb = time.perf_counter() - t0
         log(f"[bench] RPY N={N}: built in {tb:.1f}s, levels {h2.max_level + 1}, adm {h2.n_r_adm_pair}")
         Nk = h2.krnl_mat_size
+        leaf_npt = [h2.pt_cluster[2 * n + 1] - h2.pt_cluster[2 * n] + 1 for n in h2.leaf_nodes]
+        diag_pairs = float(sum(k * k for k in leaf_npt))
         X = np.asfortranarray(rng.standard_normal((Nk, nv)))
         Xd = torch.from_numpy(X.T.copy()).to(dev)          # column-major: column v at v * Nk
         Yd = torch.empty_like(Xd)
@@ -249,7 +251,12 @@ def run(args):
                 m.matvec_dev(Xd[v].data_ptr(), yv.data_ptr(), sptr); sync()
                 errs.append(_rel(Y[:, v], yv.cpu().numpy()))
             info = m.info()
-            flops = 2.0 * nv * (2 * info["U_elems"] + 2 * 9 * info["B_pairs"] + 2 * 9 * info["D_pairs"])
+            # 2 n_vec (2 sum U + 2 sum B + sum D_diag + 2 sum D_offdiag) entries of
+            # the 3x3 kernel: the diagonal D blocks act once, every other block in
+            # both directions
+            d_diag = 9.0 * diag_pairs
+            flops = 2.0 * nv * (2 * info["U_elems"] + 2 * 9 * info["B_pairs"] + d_diag
+                                + 2 * (9 * info["D_pairs"] - d_diag))
             res[mode] = {"matmuls_per_s": 1.0 / dt, "ms_per_matmul": dt * 1e3,
                          "ms_per_matmul_default_tiles": dt_default * 1e3, "autotune": tune,
                          "tuned_equals_default_bitwise": bool(np.array_equal(Y0, Y)),

@@ -65,11 +65,14 @@ class Request:
         return self.finished - self.arrival
 
 
-def request_bytes(req: Request, mode: str, prec: str) -> int:
-    """Device bytes the request holds while active."""
+def request_bytes(req: Request, mode: str, prec: str, refine: bool = False) -> int:
+    """Device bytes the request holds while active (with ``refine``, a CG
+    request also holds the FP64 handle and vectors of its refinement)."""
     n_vec = req.rhs.shape[1] if req.kind == "matmul" else 0
     b = footprint(req.h2, mode, prec, n_vec=n_vec, with_cg=req.kind == "cg")
     vec = 2 * req.rhs.size * 8            # b and x, or X and Y
+    if refine and req.kind == "cg":
+        b += footprint(req.h2, mode, "fp64") + 3 * req.rhs.size * 8   # A64, and r, d, y
     return b + vec
 
 
@@ -79,6 +82,13 @@ class _Active:
     mat: H2MatHIP
     a: object                  # device b / X
     b: object                  # device x / Y
+    # iterative refinement of an FP32-mixed CG solve (see Server)
+    mat64: H2MatHIP | None = None
+    r: object = None           # FP64 residual b - (A64 + shift) x, the rhs of a correction
+    d: object = None           # correction solve's iterate
+    y: object = None           # (A64 + shift) x
+    correcting: bool = False
+    refines: int = 0
 
 
 @dataclass
@@ -103,13 +113,31 @@ class ServeStats:
 
 
 class Server:
+    """Continuous batching of GP requests on one GPU.
+
+    CG requests on the FP32-mixed path are refined (``refine``, default on
+    for FP32-mixed): the FP32 operator's rounding is relative to |A| |x|,
+    so on a system with condition number kappa the residual its CG reaches
+    is the FP64 operator's only to ~1e-7 kappa.  After the FP32 solve the
+    server computes the residual r = b - (A64 + shift) x with an FP64 handle
+    of the same representation; while |r| > tol |b| it solves the correction
+    (A32 + shift) d = r on the FP32 path to the relative accuracy that
+    closes the gap (at most 0.5 tol |b| / |r|, with the FP32 operator's own
+    error shrinking with r) and adds d to x (at most ``max_refine`` times).
+    ``relres`` is then the FP64 residual and ``iters`` counts every FP32 CG
+    iteration.
+    """
+
     def __init__(self, hbm_budget: int, max_active: int = 64, mode: str = "jit",
-                 prec: str = "fp32mix", cg_chunk: int = 8, device: str = "cuda:0"):
+                 prec: str = "fp32mix", cg_chunk: int = 8, device: str = "cuda:0",
+                 refine: bool | None = None, max_refine: int = 4):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.mode, self.prec = mode, prec
+        self.refine = (prec == "fp32mix") if refine is None else bool(refine)
+        self.max_refine = int(max_refine)
         self.cg_chunk = int(cg_chunk)
         self.dev = torch.device(device)
         # two streams at the default priority
@@ -122,6 +150,21 @@ class Server:
         self.bt_cg.close()
         self.bt_mm.close()
 
+    def _residual(self, acts):
+        """r = b - (A64 + shift) x for the listed solves on the CG stream (one
+        co-scheduled FP64 product); returns |r| / |b| per solve."""
+        torch = self.torch
+        self.bt_cg.matvec([a.mat64 for a in acts], [a.b.data_ptr() for a in acts],
+                          [a.y.data_ptr() for a in acts], [a.req.shift for a in acts],
+                          stream=self.s_cg.cuda_stream)
+        with torch.cuda.stream(self.s_cg):
+            out = []
+            for a in acts:
+                torch.sub(a.a, a.y, out=a.r)
+                out.append(torch.linalg.vector_norm(a.r) / torch.linalg.vector_norm(a.a))
+            rho = torch.stack(out).cpu().tolist()
+        return rho
+
     # ------------------------------------------------------------------
     def _admit(self, req: Request, stream):
         torch = self.torch
@@ -130,7 +173,12 @@ class Server:
             a = torch.from_numpy(np.ascontiguousarray(req.rhs.T if req.kind == "matmul" else req.rhs))
             a = a.to(self.dev, non_blocking=False)
             b = torch.zeros_like(a)
-        return _Active(req, mat, a, b)
+        act = _Active(req, mat, a, b)
+        if self.refine and req.kind == "cg":
+            act.mat64 = H2MatHIP(req.h2, mode=self.mode, prec="fp64", host_blocks=False)
+            with torch.cuda.stream(stream):
+                act.r, act.d, act.y = torch.empty_like(a), torch.zeros_like(a), torch.empty_like(a)
+        return act
 
     def run(self, requests: list[Request], realtime: bool = True, log=None,
             log_every: float = 15.0) -> ServeStats:
@@ -140,7 +188,8 @@ class Server:
         torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in pending:
-            r.nbytes = request_bytes(r, self.mode, self.prec)
+            r.nbytes = request_bytes(r, self.mode, self.prec, self.refine)
+            r.iters = 0
         queue: deque[Request] = deque()
         cg: list[_Active] = []
         mm_inflight: list[tuple[list[_Active], object]] = []
@@ -220,24 +269,58 @@ class Server:
             # ---- poll the solves (waits for the CG stream)
             if cg:
                 done, its, rr = self.bt_cg.cg_poll([a.mat for a in cg], stream=self.s_cg.cuda_stream)
-                keep = []
+                keep, fin, ended = [], [], []
                 for a, d, it, r in zip(cg, done, its, rr):
-                    if d:
-                        a.req.result = a.b.cpu().numpy()
-                        a.req.iters, a.req.relres = it, r
-                        a.req.finished = now()
-                        st.cg_iters += it
-                        to_free.append(a)
-                        st.n_done += 1
-                    else:
+                    if not d:
                         keep.append(a)
+                        continue
+                    a.req.iters += it
+                    st.cg_iters += it
+                    if a.mat64 is None:
+                        a.req.relres = r
+                        fin.append(a)
+                    else:
+                        ended.append(a)
+                # refinement: the FP64 residuals of the FP32 solves that ended
+                if ended:
+                    with torch.cuda.stream(self.s_cg):
+                        for a in ended:
+                            if a.correcting:
+                                a.b.add_(a.d)
+                    rho = self._residual(ended)
+                    again = []
+                    for a, p in zip(ended, rho):
+                        a.correcting = False
+                        if p <= a.req.tol or a.refines >= self.max_refine or not np.isfinite(p):
+                            a.req.relres = p
+                            fin.append(a)
+                        else:
+                            a.refines += 1
+                            a.correcting = True
+                            again.append((a, min(0.5, 0.5 * a.req.tol / p)))
+                            keep.append(a)
+                    if again:
+                        with torch.cuda.stream(self.s_cg):
+                            for a, _ in again:
+                                a.d.zero_()
+                        self.bt_cg.cg_start([a.mat for a, _ in again], [a.r.data_ptr() for a, _ in again],
+                                            [a.d.data_ptr() for a, _ in again], [a.req.shift for a, _ in again],
+                                            [t for _, t in again], [a.req.max_iter for a, _ in again],
+                                            stream=self.s_cg.cuda_stream)
+                for a in fin:
+                    a.req.result = a.b.cpu().numpy()
+                    a.req.finished = now()
+                    to_free.append(a)
+                    st.n_done += 1
                 cg = keep
             # ---- free retired requests once no product is in flight (a free
             # waits for the device); their bytes return to the budget then
             if to_free and not mm_inflight:
                 for a in to_free:
                     a.mat.close()
-                    a.a = a.b = None
+                    if a.mat64 is not None:
+                        a.mat64.close()
+                    a.a = a.b = a.r = a.d = a.y = None
                     used -= a.req.nbytes
                 to_free = []
             # ---- only products in flight: wait for the oldest one
@@ -249,6 +332,8 @@ class Server:
         torch.cuda.synchronize()
         for a in to_free:
             a.mat.close()
+            if a.mat64 is not None:
+                a.mat64.close()
         st.makespan = now()
         lat = sorted(r.latency for r in requests if not r.rejected)
         st.latencies = lat

@@ -69,7 +69,7 @@ def test_serving_fp32mix_against_fp64(reqs):
 
 def test_admission_respects_the_budget(reqs):
     rs = _clone(reqs)
-    sizes = sorted(request_bytes(r, "jit", "fp32mix") for r in rs)
+    sizes = sorted(request_bytes(r, "jit", "fp32mix", refine=True) for r in rs)
     budget = sizes[-1] + sizes[0]     # room for the largest plus the smallest
     st = Server(hbm_budget=budget, max_active=8, prec="fp32mix").run(rs, realtime=False)
     assert st.n_done == len(rs) and st.peak_bytes <= budget
