@@ -80,6 +80,10 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: most requests on the GPU at once")
     ap.add_argument("--cg-max-iter", type=int, default=3000, help="serving: CG iteration cap")
     ap.add_argument("--build-workers", type=int, default=4, help="serving: host construction processes")
+    ap.add_argument("--serve-legs", default="both", choices=["both", "batched", "alone"],
+                    help="serving: both legs on prebuilt requests, or one leg on requests built while served")
+    ap.add_argument("--build-window", type=int, default=64,
+                    help="serving, one leg: most requests built but not yet admitted")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="gaussian1m: requests of the serving leg (0: no serving leg)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="gaussian1m: serving leg arrivals per second")
@@ -98,13 +102,6 @@ def main():
     from h2pack_amd import construct as C
     from h2pack_amd.h2mat import H2MatHIP, direct_nbody
 
-    if args.workload != "gaussian1m":
-        import bench_workloads
-        bench_workloads.run(args)
-        return
-
-    N, K, W = args.n, args.steps, args.warmup
-    sigma2 = 1e-2
     if args.copy_calib:
         src = torch.ones(2 << 27, dtype=torch.float64, device="cuda:0")   # 2 GiB
         dst = torch.empty_like(src)
@@ -112,6 +109,14 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copy calibration: 2 GiB read + 2 GiB written by one copy kernel")
         del src, dst
+
+    if args.workload != "gaussian1m":
+        import bench_workloads
+        bench_workloads.run(args)
+        return
+
+    N, K, W = args.n, args.steps, args.warmup
+    sigma2 = 1e-2
     t0 = time.perf_counter()
     coord = C.uniform_cube_points(N, 10.0, args.seed)
     h2 = C.build_h2(coord, C.KRNL_GAUSSIAN, [1.0], 1e-6, 300, BD_JIT=1, pp_kind="numerical",
@@ -143,7 +148,8 @@ def main():
         log(f"[bench] {label}: autotuned block launch -> variant {r['variants'][0]} "
             f"(default {r['default_ms']:.3f} ms, chosen {r['tuned_ms']:.3f} ms, {r['tune_s']:.1f}s)"
             + ("" if "aot" in label else
-               f"; D blocks {'beside the sweeps' if r['d_concurrent'] else 'in order'}"))
+               f"; D blocks {'beside the sweeps' if r['d_concurrent'] else 'in order'}"
+               f"; level split {'on' if r['level_split'] else 'off'}"))
         return r
 
     def timed(m, label):
@@ -203,7 +209,7 @@ def main():
             "rel_err_vs_fp64_gpu": err_f_fp64,
             "rel_err_vs_direct_nbody_1000rows": err_f_direct,
             "kernel_ms": info_f["t_blocks"], "block_variant": tune_f["variants"][0],
-            "d_concurrent": tune_f["d_concurrent"]}
+            "d_concurrent": tune_f["d_concurrent"], "level_split": tune_f["level_split"]}
     log(f"[bench] fp32mix-jit: {K / el_f:.2f} matvec/s ({el_j / el_f:.2f}x FP64); rel err vs FP64 GPU "
         f"{err_f_fp64:.3e}, vs direct n-body {err_f_direct:.3e}")
 
@@ -325,7 +331,7 @@ def main():
                      "kernel": ("k_blocks_jit2_occ<1, double, 4, 0>" if jv == 0
                                 else "k_blocks_jit2<1, double, 0>"), "kernel_ms": info_j["t_blocks_b"],
                      "flops_per_launch": flops_b, "block_variant": jv,
-                     "d_concurrent": tune_j["d_concurrent"],
+                     "d_concurrent": tune_j["d_concurrent"], "level_split": tune_j["level_split"],
                      "d_launch": {"kernel": ("k_blocks_jit2_occ<1, double, 4, 1>" if jv == 0
                                              else "k_blocks_jit2<1, double, 1>"),
                                   "kernel_ms": info_j["t_blocks_d"], "flops_per_launch": flops_d,

@@ -57,7 +57,7 @@ class Info(ct.Structure):
         ("t_bwd", ct.c_double), ("t_final", ct.c_double), ("t_total", ct.c_double),
         ("n_profiled", ct.c_int), ("mv_variant", ct.c_int),
         ("t_blocks_b", ct.c_double), ("t_blocks_d", ct.c_double),
-        ("mv_d_concurrent", ct.c_int),
+        ("mv_d_concurrent", ct.c_int), ("mv_level_split", ct.c_int),
     ]
 
     def as_dict(self):

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -133,15 +135,26 @@ struct h2p_hip_handle {
     bool mm_ok = false;
     std::vector<Level> fwd_levels, bwd_levels;
     int n_blk = 0, n_red = 0, n_leaf = 0;
-    // grouped block launches: part 0 = B blocks (all blocks when not split),
-    // parts 1 and 2 = D blocks launched before and after the B blocks; each
-    // part a contiguous range of d_blk with its workgroup-unit tasks first
+    // grouped block launches: part 0 = B blocks (all blocks when not split;
+    // on a split handle the B blocks of the deep levels), parts 1 and 2 = D
+    // blocks launched before and after the B blocks, part 3 = the B blocks
+    // of the shallow levels; each part a contiguous range of d_blk with its
+    // workgroup-unit tasks first
     struct BlkPart { int off = 0, n = 0, big = 0, units = 0; };
-    BlkPart part[3];
+    BlkPart part[4];
     bool split = false;             // D blocks on aux_stream, concurrent with the sweeps
     bool d_serial = false;          // split handle whose D blocks run in order (h2p_hip_autotune)
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr;
+    // level split of a split handle: the levels above lv_cut (upward sweep,
+    // their B blocks, skeleton sums and downward sweep) run on lv_stream
+    // while the deep levels' B blocks run; lv_on = 0 runs them in order
+    // (h2p_hip_autotune).  Shallow entries come first in red, deep ones
+    // first in fwd_levels.
+    int lv_cut = 0, lv_on = 0, lv_possible = 0;
+    int n_fwd_deep = 0, n_bwd_shallow = 0, n_red_shallow = 0;
+    hipStream_t lv_stream = nullptr;
+    hipEvent_t ev_lv = nullptr, ev_side = nullptr;
     long long arena_len = 0;
     size_t device_bytes = 0;
     h2p_hip_info info{};
@@ -198,6 +211,9 @@ static void free_handle(h2p_hip_handle *h) {
     if (h->graph) (void)hipGraphDestroy(h->graph);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     if (h->aux_stream) (void)hipStreamDestroy(h->aux_stream);
+    if (h->lv_stream) (void)hipStreamDestroy(h->lv_stream);
+    if (h->ev_lv) (void)hipEventDestroy(h->ev_lv);
+    if (h->ev_side) (void)hipEventDestroy(h->ev_side);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_mid) (void)hipEventDestroy(h->ev_mid);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
@@ -242,8 +258,9 @@ static void launch_jit2_part(h2p_hip_handle *h, hipStream_t s, const h2p_hip_han
 
 template <int KID, typename T>
 static void launch_jit2(h2p_hip_handle *h, hipStream_t s, int part) {
-    // the D parts share one kernel instantiation (template argument 1)
-    if (part == 0) launch_jit2_part<KID, T, 0>(h, s, h->part[0]);
+    // the B parts (0, 3) share one kernel instantiation (template argument
+    // 0), the D parts (1, 2) another
+    if (part == 0 || part == 3) launch_jit2_part<KID, T, 0>(h, s, h->part[part]);
     else launch_jit2_part<KID, T, 1>(h, s, h->part[part]);
 }
 
@@ -308,24 +325,64 @@ static void launch_blocks(h2p_hip_handle *h, hipStream_t s, int part) {
 // handle the D blocks, which read only px, run on aux_stream and are joined
 // at the end of the interior, so that their kernel evaluations overlap the
 // HBM-bound sweeps: part 1 from the start of the interior (beside the
-// upward sweep), part 2 once the B blocks are done (beside the skeleton
-// sums, the downward sweep and the leaves' U y1 rows, k_leaf_u, which ends
-// the interior of a split handle; k_leaf_d then adds the D contributions).
+// upward sweep), part 2 once the deep B blocks are done (beside the
+// skeleton sums, the downward sweep and the leaves' U y1 rows, k_leaf_u,
+// which ends the interior of a split handle; k_leaf_d then adds the D
+// contributions).  With the level split on (lv_on), the levels above lv_cut
+// form a chain of their own on lv_stream once the deep levels' upward sweep
+// is done -- their upward sweep, their B blocks (part 3), their skeleton
+// sums and their downward sweep -- beside the deep levels' B blocks (part 0)
+// on s; s then adds the deep skeleton sums, waits for the chain and runs the
+// deep downward sweep.  Every kernel and every sum is the same in any
+// schedule, so the results are bitwise the same.
 // With profiling events (ev: 7 per call) every phase runs in order on s
 // instead: ev[1] after the upward sweep, ev[2] after the B blocks, ev[3]
 // after the D blocks, ev[4] after the skeleton sums, ev[5] after the
 // downward sweep.
+static void launch_fwd(h2p_hip_handle *h, hipStream_t s, int l0, int l1) {
+    for (int k = l0; k < l1; k++) {
+        const Level &L = h->fwd_levels[k];
+        hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
+    }
+}
+
+static void launch_bwd(h2p_hip_handle *h, hipStream_t s, int l0, int l1) {
+    for (int k = l0; k < l1; k++) {
+        const Level &L = h->bwd_levels[k];
+        hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
+    }
+}
+
+static void launch_reduce(h2p_hip_handle *h, hipStream_t s, int t0, int t1) {
+    if (t1 > t0)
+        hipLaunchKernelGGL(k_reduce_y1, dim3(t1 - t0), dim3(256), 0, s, h->d_red + t0, h->d_contrib, h->d_arena,
+                           h->d_y1);
+}
+
 static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
     const bool fork = h->split && !h->d_serial && !ev && (h->part[1].n > 0 || h->part[2].n > 0);
+    const bool lv = h->split && h->lv_on && h->lv_possible && !ev;
+    const int nf = (int)h->fwd_levels.size(), nbw = (int)h->bwd_levels.size();
+    const int fd = lv ? h->n_fwd_deep : nf, bs = lv ? h->n_bwd_shallow : 0, rs = lv ? h->n_red_shallow : 0;
     if (fork) {
         (void)hipEventRecord(h->ev_fork, s);
         (void)hipStreamWaitEvent(h->aux_stream, h->ev_fork, 0);
         launch_blocks(h, h->aux_stream, 1);
     }
-    for (const Level &L : h->fwd_levels)
-        hipLaunchKernelGGL(k_fwd2, dim3(L.n), dim3(256), 0, s, h->d_fwdt + L.off, h->d_U, h->d_xin);
+    launch_fwd(h, s, 0, fd);
+    if (lv) {
+        hipStream_t q = h->lv_stream;
+        (void)hipEventRecord(h->ev_lv, s);
+        (void)hipStreamWaitEvent(q, h->ev_lv, 0);
+        launch_fwd(h, q, fd, nf);
+        launch_blocks(h, q, 3);
+        launch_reduce(h, q, 0, rs);
+        launch_bwd(h, q, 0, bs);
+        (void)hipEventRecord(h->ev_side, q);
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
     launch_blocks(h, s, 0);
+    if (!lv) launch_blocks(h, s, 3);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (fork) {
         if (h->part[2].n > 0) {
@@ -339,14 +396,13 @@ static void launch_interior(h2p_hip_handle *h, hipStream_t s, hipEvent_t *ev) {
         launch_blocks(h, s, 2);
     }
     if (ev) (void)hipEventRecord(ev[3], s);
-    if (h->n_red > 0)
-        hipLaunchKernelGGL(k_reduce_y1, dim3(h->n_red), dim3(256), 0, s, h->d_red, h->d_contrib, h->d_arena, h->d_y1);
-    if (h->periodic)
+    launch_reduce(h, s, rs, h->n_red);
+    if (h->periodic)   // (periodic handles are not split)
         hipLaunchKernelGGL(k_root_block, dim3((h->per_r + 3) / 4), dim3(256), 0, s, h->d_per, h->per_r,
                            h->d_xin + h->N + h->per_soff, h->d_y1 + h->per_soff);
     if (ev) (void)hipEventRecord(ev[4], s);
-    for (const Level &L : h->bwd_levels)
-        hipLaunchKernelGGL(k_bwd2, dim3(L.n), dim3(256), 0, s, h->d_bwdt + L.off, h->d_UT, h->d_y1, h->d_yp);
+    if (lv) (void)hipStreamWaitEvent(s, h->ev_side, 0);
+    launch_bwd(h, s, bs, nbw);
     if (ev) (void)hipEventRecord(ev[5], s);
     if (h->split)
         hipLaunchKernelGGL(k_leaf_u, dim3(h->n_leaf), dim3(256), 0, s, h->d_leaf, h->d_contrib, h->d_arena, h->d_UT,
@@ -781,6 +837,18 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         h->wide = !(arg < 0x1p30);   // also for non-finite coordinates or parameters
     }
     // ---- upward sweep tasks per level
+    // level split (split handles): the tree level holding the most leaves
+    // (the deepest on a tie) and everything below it are "deep"
+    {
+        std::vector<int> lcount(g->max_level + 1, 0);
+        for (int i = 0; i < nlf; i++) {
+            const int lv = g->node_level[g->leaf_nodes[i]];
+            if (lv >= 0 && lv <= g->max_level) lcount[lv]++;
+        }
+        h->lv_cut = g->max_level;
+        for (int lv = g->max_level; lv >= 0; lv--)
+            if (lcount[lv] > lcount[h->lv_cut]) h->lv_cut = lv;
+    }
     std::vector<FwdTask> fwdt;
     for (int lv = g->max_level; lv >= g->min_adm_level; lv--) {
         Level L{(int)fwdt.size(), 0};
@@ -807,10 +875,14 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             }
         }
         L.n = (int)fwdt.size() - L.off;
-        if (L.n > 0) h->fwd_levels.push_back(L);
+        if (L.n > 0) {
+            h->fwd_levels.push_back(L);
+            if (lv >= h->lv_cut) h->n_fwd_deep++;
+        }
     }
     // ---- grouped block tasks (B then D) and partial destinations
     std::vector<BlkTask> blk;
+    std::vector<char> blk_deep;   // B tasks: skeleton side at level >= lv_cut
     std::vector<std::vector<long long>> dest_y1(nn), dest_leaf(nn);
     long long arena = 0;
     size_t btot = (g->n_r_adm_pair > 0) ? g->B_ptr[g->n_r_adm_pair] : 0;
@@ -884,8 +956,18 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         }
         B_pairs += (double)t.n0 * t.n1;
         blk.push_back(t);
+        blk_deep.push_back(std::max(l0, l1) >= h->lv_cut ? 1 : 0);
     }
     const int nB = (int)blk.size();
+    // the deep B blocks first, the shallow ones after them (stable)
+    int nB_deep = nB;
+    {
+        std::vector<BlkTask> deep, shallow;
+        for (int i = 0; i < nB; i++) (blk_deep[i] ? deep : shallow).push_back(blk[i]);
+        nB_deep = (int)deep.size();
+        std::copy(deep.begin(), deep.end(), blk.begin());
+        std::copy(shallow.begin(), shallow.end(), blk.begin() + nB_deep);
+    }
     for (int i = 0; i < nD; i++) {
         BlkTask t{};
         int n0, n1;
@@ -927,10 +1009,12 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     const bool tiled = mode == H2P_HIP_JIT && h->kid != KR &&
                        (prec == H2P_HIP_FP32MIX || h->kid == KG || h->kid == KM);
     for (auto &P : h->part) P = h2p_hip_handle::BlkPart{};
-    h->part[0].n = h->split ? nB : h->n_blk;
-    h->part[1].off = h->part[0].n;
-    h->part[1].n = h->n_blk - h->part[0].n;
-    for (int k = 0; k < 2; k++) {
+    h->part[0].n = h->split ? nB_deep : h->n_blk;
+    h->part[3].off = h->part[0].n;
+    h->part[3].n = h->split ? nB - nB_deep : 0;
+    h->part[1].off = h->split ? nB : h->n_blk;
+    h->part[1].n = h->n_blk - h->part[1].off;
+    for (int k : {0, 3, 1}) {
         auto &P = h->part[k];
         auto b0 = blk.begin() + P.off, b1 = b0 + P.n;
         std::stable_sort(b0, b1, [](const BlkTask &x, const BlkTask &y) {
@@ -977,18 +1061,23 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             if (!active[n]) has_push[n] = 0;
         }
     // ---- non-leaf skeleton sums (leaves sum their own in the output kernel)
+    // (the nodes above lv_cut first: on a split handle their sums belong to
+    // the shallow levels' chain)
     std::vector<long long> contrib;
     std::vector<RedTask> red;
-    for (int n = 0; n < nn; n++) {
-        if (g->U_ncol[n] == 0 || g->n_child[n] == 0) continue;
-        RedTask t{};
-        t.dst = soff[n];
-        t.len = g->U_ncol[n];
-        t.cbeg = (int)contrib.size();
-        for (long long o : dest_y1[n]) contrib.push_back(o);
-        t.cend = (int)contrib.size();
-        red.push_back(t);
-    }
+    for (int pass = 0; pass < 2; pass++)
+        for (int n = 0; n < nn; n++) {
+            if (g->U_ncol[n] == 0 || g->n_child[n] == 0) continue;
+            if ((g->node_level[n] < h->lv_cut) != (pass == 0)) continue;
+            RedTask t{};
+            t.dst = soff[n];
+            t.len = g->U_ncol[n];
+            t.cbeg = (int)contrib.size();
+            for (long long o : dest_y1[n]) contrib.push_back(o);
+            t.cend = (int)contrib.size();
+            red.push_back(t);
+            if (pass == 0) h->n_red_shallow++;
+        }
     h->n_red = (int)red.size();
     // ---- downward sweep: one task per 256-row chunk, level by level
     std::vector<BwdTask2> bwdt;
@@ -1011,7 +1100,10 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             }
         }
         L.n = (int)bwdt.size() - L.off;
-        if (L.n > 0) h->bwd_levels.push_back(L);
+        if (L.n > 0) {
+            h->bwd_levels.push_back(L);
+            if (lv < h->lv_cut) h->n_bwd_shallow++;
+        }
     }
     std::vector<LeafTask2> leaf;
     for (int i = 0; i < nlf; i++) {
@@ -1257,6 +1349,70 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             if (!ok) return bail(fail(H2P_HIP_EINVAL, "matmul task out of range"));
         }
     }
+    // ---- level split (after the bounds checks above): the two chains touch
+    // what the schedule assumes.
+    // The shallow chain (upward sweep above lv_cut, part 3, the first
+    // n_red_shallow skeleton sums, the first n_bwd_shallow downward levels)
+    // runs beside part 0 after the deep upward sweep; then s adds the deep
+    // skeleton sums and runs the deep downward levels.  Checked here: every
+    // skeleton coefficient a B block reads comes from its own chain's upward
+    // sweep (or from the deep one, which both chains wait for), every
+    // partial a skeleton sum adds comes from its own chain's B blocks, and
+    // every skeleton sum a downward task reads comes from its own chain.
+    h->lv_possible = (h->split && h->n_fwd_deep < (int)h->fwd_levels.size()) ? 1 : 0;
+    if (h->lv_possible) {
+        const int fd = h->n_fwd_deep, bs = h->n_bwd_shallow, rs = h->n_red_shallow;
+        // y0 entries: 1 = deep upward sweep, 2 = shallow upward sweep
+        std::vector<char> y0c((size_t)S, 0), y1c((size_t)S, 0);
+        for (int k = 0; k < (int)h->fwd_levels.size(); k++) {
+            const Level &L = h->fwd_levels[k];
+            for (int i = L.off; i < L.off + L.n; i++) {
+                const FwdTask &t = fwdt[i];
+                for (int c = t.col0; c < std::min(t.r, t.col0 + 4 * H2P_WAVE); c++)
+                    y0c[(size_t)(t.out_off - N + c)] = k < fd ? 1 : 2;
+            }
+        }
+        // partial ranges of the B blocks: (start, length, chain)
+        std::vector<std::array<long long, 3>> pr;
+        auto y0_ok = [&](int off, int len, int chain) {
+            if (off < N) return true;   // px
+            for (int e = 0; e < len; e++) {
+                const char c = y0c[(size_t)(off - N + e)];
+                if (c != 1 && c != chain) return false;
+            }
+            return true;
+        };
+        bool ok = true;
+        for (int k : {0, 3}) {
+            const int chain = k == 0 ? 1 : 2;
+            const auto &P = h->part[k];
+            for (int i = P.off; i < P.off + P.n && ok; i++) {
+                const BlkTask &t = blk[i];
+                ok = y0_ok(t.in0, t.ncol, chain) && (!(t.flags & 1) || y0_ok(t.in1, t.nrow, chain));
+                pr.push_back({t.out0, (long long)t.nrow, chain});
+                if (t.flags & 1) pr.push_back({t.out1, (long long)t.ncol, chain});
+            }
+        }
+        std::sort(pr.begin(), pr.end());
+        auto chain_of = [&](long long o) {
+            auto it = std::upper_bound(pr.begin(), pr.end(), std::array<long long, 3>{o, LLONG_MAX, LLONG_MAX});
+            if (it == pr.begin()) return 0LL;
+            --it;
+            return o < (*it)[0] + (*it)[1] ? (*it)[2] : 0LL;
+        };
+        for (int i = 0; i < h->n_red && ok; i++) {
+            const RedTask &t = red[i];
+            const int chain = i < rs ? 2 : 1;
+            for (int c = t.cbeg; c < t.cend && ok; c++) ok = chain_of(contrib[c]) == chain;
+            for (int e = 0; e < t.len; e++) y1c[(size_t)t.dst + e] = (char)chain;
+        }
+        for (int k = 0; k < (int)h->bwd_levels.size() && ok; k++) {
+            const Level &L = h->bwd_levels[k];
+            const int chain = k < bs ? 2 : 1;
+            for (int i = L.off; i < L.off + L.n && ok; i++) ok = y1c[(size_t)bwdt[i].y1_off] == chain;
+        }
+        if (!ok) return bail(fail(H2P_HIP_EINVAL, "level split: a task reads data of the other chain"));
+    }
     if (per && (g->n_child[root] == 0 || g->U_ncol[root] == 0))
         return bail(fail(H2P_HIP_EINVAL, "periodic mode needs a root with children and a basis"));
     const size_t per_elems = per ? (size_t)g->U_ncol[root] * g->U_ncol[root] : 0;
@@ -1362,6 +1518,12 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             hipEventCreateWithFlags(&h->ev_mid, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
             return bail(fail(H2P_HIP_EDEVICE, "auxiliary stream / event create failed"));
+        if (h->lv_possible &&
+            (hipStreamCreate(&h->lv_stream) != hipSuccess ||
+             hipEventCreateWithFlags(&h->ev_lv, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&h->ev_side, hipEventDisableTiming) != hipSuccess))
+            return bail(fail(H2P_HIP_EDEVICE, "level-split stream / event create failed"));
+        h->lv_on = h->lv_possible;
     }
     // the upload is synchronous: callers may use the handle on any stream,
     // including streams that do not order with the null stream
@@ -1699,6 +1861,16 @@ static int time_interior(h2p_hip_handle *h, int reps, double *ms) {
 // whether its D blocks run beside the sweeps on the second stream or in
 // order (concurrency pays when the D blocks cover the sweeps, and can cost
 // time where the sweeps are long and the D blocks few).  Same results.
+// Schedule of a split handle's matvec (bit 0: D blocks in order, bit 1:
+// level split off); the captured graph is dropped when it changes.
+static void set_schedule(h2p_hip_handle *h, int k) {
+    const bool ser = h->split && (k & 1);
+    const int lvon = (h->lv_possible && !(k & 2)) ? 1 : 0;
+    if (h->d_serial != ser || h->lv_on != lvon) drop_matvec_graph(h);
+    h->d_serial = ser;
+    h->lv_on = lvon;
+}
+
 static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tuned_ms) {
     if (t_default_ms) *t_default_ms = 0.0;
     if (t_tuned_ms) *t_tuned_ms = 0.0;
@@ -1719,9 +1891,11 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
         if (it != g_tune_cache.end()) {
             const int v = it->second & 1;
             const bool ser = h->split && (it->second & 2);
-            if (h->mv_var != v || h->d_serial != ser) drop_matvec_graph(h);
+            const int lvon = (h->lv_possible && !(it->second & 4)) ? 1 : 0;
+            if (h->mv_var != v || h->d_serial != ser || h->lv_on != lvon) drop_matvec_graph(h);
             h->mv_var = v;
             h->d_serial = ser;
+            h->lv_on = lvon;
             return 0;
         }
     }
@@ -1734,10 +1908,10 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
     int err = 0;
     for (int v = 0; v < 2 && !err; v++) {
         h->mv_var = v;
-        for (int k = 0; k < 3; k++) launch_blocks(h, s, k);   // warm-up
+        for (int k = 0; k < 4; k++) launch_blocks(h, s, k);   // warm-up
         if (hipEventRecord(e0, s) != hipSuccess) { err = 1; break; }
         for (int r = 0; r < reps; r++)
-            for (int k = 0; k < 3; k++) launch_blocks(h, s, k);
+            for (int k = 0; k < 4; k++) launch_blocks(h, s, k);
         float ms = 0.f;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
@@ -1755,18 +1929,20 @@ static int autotune_matvec(h2p_hip_handle *h, double *t_default_ms, double *t_tu
     if (t_tuned_ms) *t_tuned_ms = t[best];
     int code = best;
     if (h->split) {
-        double ti[2] = {0.0, 0.0};   // D blocks concurrent, in order
-        for (int k = 0; k < 2; k++) {
-            const bool ser = k == 1;
-            if (h->d_serial != ser) drop_matvec_graph(h);
-            h->d_serial = ser;
-            int rc = time_interior(h, 3, &ti[k]);
+        // schedules: bit 0 = D blocks in order (else beside the sweeps),
+        // bit 1 = level split off (else the shallow levels' chain runs
+        // beside the deep B blocks)
+        const int nsch = h->lv_possible ? 4 : 2;
+        double ti[4] = {0.0, 0.0, 0.0, 0.0};
+        int bk = 0;
+        for (int k = 0; k < nsch; k++) {
+            set_schedule(h, k);
+            int rc = time_interior(h, 8, &ti[k]);
             if (rc) return rc;
+            if (ti[k] < ti[bk]) bk = k;
         }
-        const bool ser = ti[1] < ti[0];
-        if (h->d_serial != ser) drop_matvec_graph(h);
-        h->d_serial = ser;
-        code |= ser ? 2 : 0;
+        set_schedule(h, bk);
+        code |= ((bk & 1) ? 2 : 0) | ((bk & 2) ? 4 : 0);
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[sig] = code;
@@ -1837,6 +2013,12 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
 
 int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant) {
     if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
+    if (phase == 5) {   // the matvec's schedule on a split handle
+        if (variant < -1 || variant > 3) return fail(H2P_HIP_EINVAL, "bad matvec schedule");
+        if (variant < 0) return (h->d_serial ? 1 : 0) | (h->lv_on ? 0 : 2);
+        set_schedule(h, variant);
+        return 0;
+    }
     if (phase == 4) {   // the matvec's block launch
         if (variant < -1 || variant > 1) return fail(H2P_HIP_EINVAL, "bad matvec variant");
         if (variant < 0) return h->mv_var;
@@ -1901,6 +2083,7 @@ int h2p_hip_get_info(h2p_hip_handle *h, h2p_hip_info *info) {
     }
     h->info.mv_variant = h->mv_var;
     h->info.mv_d_concurrent = (h->split && !h->d_serial) ? 1 : 0;
+    h->info.mv_level_split = (h->split && h->lv_on && h->lv_possible) ? 1 : 0;
     *info = h->info;
     return 0;
 }

@@ -171,8 +171,10 @@ class H2MatHIP:
     def autotune(self, n_vec: int) -> dict:
         """Time the kernel variants on this matrix and keep the fastest
         (h2p_hip_autotune): the matmul tiles per phase for ``n_vec`` columns,
-        or with ``n_vec`` = 0 the matvec's grouped block launch and whether
-        its D blocks run beside the sweeps (``d_concurrent``)."""
+        or with ``n_vec`` = 0 the matvec's grouped block launch, whether its
+        D blocks run beside the sweeps (``d_concurrent``) and whether the
+        levels above the leaf level run as a chain of their own beside the
+        deep B blocks (``level_split``)."""
         t0, t1 = ct.c_double(0), ct.c_double(0)
         _check(self.lib.h2p_hip_autotune(self.handle, int(n_vec), ct.byref(t0), ct.byref(t1)))
         out = {"default_ms": t0.value, "tuned_ms": t1.value}
@@ -180,6 +182,7 @@ class H2MatHIP:
             inf = self.info()
             out["variants"] = [inf["mv_variant"]]
             out["d_concurrent"] = inf["mv_d_concurrent"]
+            out["level_split"] = inf["mv_level_split"]
         else:
             out["variants"] = [self.mm_variant(ph, n_vec) for ph in range(4)]
         return out

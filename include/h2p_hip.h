@@ -136,6 +136,11 @@ typedef struct {
      * second stream beside the sweeps, 0 when they run in order
      * (h2p_hip_autotune) */
     int mv_d_concurrent;
+    /* 1 when the levels above the leaf level (their upward sweep, B blocks,
+     * skeleton sums and downward sweep) run on a third stream beside the
+     * deep levels' B blocks, 0 when everything runs in order
+     * (h2p_hip_autotune) */
+    int mv_level_split;
 } h2p_hip_info;
 
 /* Mirror an H2 representation into HBM.  mode = H2P_HIP_AOT / _JIT,
@@ -205,7 +210,10 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms,
 /* Pin the tile variant (0 .. H2P_HIP_MM_VARIANTS-1) of one matmul phase
  * (0..3, or -1 for all) for panels of n_vec columns; variant -1 returns the
  * current choice of the phase (phase -1: of phase 0) instead.  Phase 4 is
- * the matvec's block launch (variants 0 and 1; n_vec is ignored). */
+ * the matvec's block launch (variants 0 and 1; n_vec is ignored), phase 5
+ * the matvec's schedule on JIT handles of the scalar kernels (bit 0: D
+ * blocks in order instead of beside the sweeps, bit 1: no level split;
+ * every schedule gives bitwise the same y). */
 int h2p_hip_mm_variant(h2p_hip_handle *h, int phase, int n_vec, int variant);
 
 /* Record per-phase HIP events on each call (up to 512 calls between reads;

@@ -19,6 +19,10 @@ SPECS = {
     "matern32_leaf300": (C.KRNL_MATERN32, [1.0], 17000, 10.0, 300, 1e-6, "numerical"),
     "rpy_leaf300": (C.KRNL_RPY, [1.0], 17000, 0.03, 300, 1e-7, "surface"),
     "rpy": (C.KRNL_RPY, [1.0], 2500, 0.03, 120, 1e-7, "surface"),
+    # deeper trees (leaves on two levels, admissible pairs on three or more
+    # levels): the matvec's level split has levels above the leaf level
+    "gaussian_deep": (C.KRNL_GAUSSIAN, [1.0], 60000, 10.0, 100, 1e-6, "surface"),
+    "coulomb_deep": (C.KRNL_COULOMB, [], 60000, 1.0, 100, 1e-6, "surface"),
     # tiny trees for dense assembly
     "tiny_coulomb": (C.KRNL_COULOMB, [], 900, 1.0, 40, 1e-8, "surface"),
     "tiny_gaussian": (C.KRNL_GAUSSIAN, [0.5], 900, 4.0, 40, 1e-8, "numerical"),

@@ -347,3 +347,34 @@ def test_periodic_matvec_and_matmul_match_oracle():
     with pytest.raises(H2PHipError) as e:
         H2MatHIP(h2, mode="aot")
     assert e.value.code == abi.ENOSYS
+
+
+@pytest.mark.parametrize("name,prec", [("gaussian_deep", "fp64"), ("gaussian_deep", "fp32mix"),
+                                       ("coulomb_deep", "fp64")])
+def test_matvec_schedules_agree(name, prec, monkeypatch):
+    """A JIT handle of the scalar kernels has four matvec schedules (D blocks
+    beside the sweeps or in order; the levels above the leaf level as a chain
+    of their own on a third stream beside the deep levels' B blocks, or not):
+    every schedule, as a captured graph and eagerly, gives bitwise the y of
+    the profiled path, which runs every launch in order on one stream."""
+    h2, _ = problem(name)
+    g = H2MatHIP(h2, mode="jit", prec=prec)
+    monkeypatch.setenv("H2P_HIP_NO_GRAPH", "1")
+    e = H2MatHIP(h2, mode="jit", prec=prec)
+    monkeypatch.delenv("H2P_HIP_NO_GRAPH")
+    x = np.random.default_rng(29).random(h2.krnl_mat_size) - 0.5
+    g.set_profiling(True)
+    y_serial = g.matvec(x)
+    g.set_profiling(False)
+    assert g.info()["mv_level_split"] == 1      # the default where the tree allows it
+    for k in range(4):
+        g.mm_variant(5, 0, k)
+        e.mm_variant(5, 0, k)
+        assert g.mm_variant(5, 0) == k
+        assert g.info()["mv_level_split"] == (0 if k & 2 else 1)
+        for it in range(2):
+            assert np.array_equal(g.matvec(x), y_serial), (k, it)
+        assert np.array_equal(e.matvec(x), y_serial), k
+    ref = O.OracleH2(h2, BD_JIT=1).matvec(x, n_thread=8)
+    bar = 1e-12 if prec == "fp64" else 1e-6
+    assert rel(y_serial, ref) < bar
