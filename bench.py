@@ -84,6 +84,8 @@ def main():
                     help="serving: both legs on prebuilt requests, or one leg on requests built while served")
     ap.add_argument("--build-window", type=int, default=64,
                     help="serving, one leg: most requests built but not yet admitted")
+    ap.add_argument("--build-head", type=int, default=16,
+                    help="serving, one leg: requests built before the clock starts")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="gaussian1m: requests of the serving leg (0: no serving leg)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="gaussian1m: serving leg arrivals per second")

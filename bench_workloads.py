@@ -87,6 +87,43 @@ def start_request_builds(n, seed, rate, cg_max_iter, workers):
     return ex, futs
 
 
+def start_stream(n, seed, rate, cg_max_iter, workers, window):
+    """Requests built while they are served: placeholders in arrival order,
+    each given a build future by a feeder thread that keeps at most
+    ``window`` requests built or building but not yet admitted by the server
+    (so host memory stays bounded).  Call before the GPU is first used."""
+    import math as _m
+    import multiprocessing as mp
+    import threading
+    from concurrent.futures import ProcessPoolExecutor
+    from h2pack_amd.serve import Request, request_specs
+    specs = request_specs(n, seed, 10_000, 400_000, rate)
+    kw = {"max_iter": cg_max_iter}
+    workers = max(1, workers)
+    threads = max(1, 16 // workers)
+    # the workers start at the feeder's first submit and inherit these then
+    # (bounded BLAS / OpenMP threads that sleep when idle), so they stay set
+    # in this process, which builds nothing itself
+    os.environ.update({"OMP_WAIT_POLICY": "PASSIVE", "OPENBLAS_NUM_THREADS": str(threads),
+                       "H2P_BUILD_THREADS": str(threads)})
+    ex = ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
+    reqs = [Request(rid=s["rid"], h2=None, kind=s["kind"], rhs=None, arrival=s["arrival"]) for s in specs]
+
+    def feed():
+        k = 0
+        while k < len(specs):
+            waiting = sum(1 for r in reqs[:k] if _m.isnan(r.admitted) and not r.rejected)
+            if waiting >= window:
+                time.sleep(0.02)
+                continue
+            reqs[k].future = ex.submit(_build_worker, specs[k], kw, threads)
+            k += 1
+
+    th = threading.Thread(target=feed, daemon=True)
+    th.start()
+    return reqs, ex, th
+
+
 def collect_requests(ex, futs):
     reqs = [None] * len(futs)
     for rid, f in futs:
@@ -95,6 +132,31 @@ def collect_requests(ex, futs):
         log(f"[bench] request {r.rid}: {r.kind} N={r.N} built in {tb:.1f}s")
     ex.shutdown()
     return reqs
+
+
+def serve_one(reqs, name, prec, slots, hbm_frac=0.8, drop_host=False):
+    """One serving leg over ``reqs``; returns its summary dict."""
+    import torch
+    from h2pack_amd.serve import Server
+    free, total = torch.cuda.mem_get_info()
+    budget = int(hbm_frac * free)
+    srv = Server(hbm_budget=budget, max_active=slots, prec=prec, cg_chunk=8, drop_host=drop_host)
+    log(f"[bench] serving {len(reqs)} requests: {name} ({prec}, up to {slots} at once)")
+    st = srv.run(reqs, realtime=True, log=log)
+    srv.close()
+    cg = [r for r in reqs if r.kind == "cg"]
+    d = st.as_dict()
+    d["cg_converged"] = sum(1 for r in cg if r.relres <= r.tol)
+    d["cg_requests"] = len(cg)
+    d["cg_mean_iters"] = float(np.mean([r.iters for r in cg])) if cg else 0.0
+    late = [r.loaded - r.arrival for r in reqs if r.loaded == r.loaded]
+    d["build_late_max_s"] = float(max(0.0, max(late))) if late else 0.0
+    d["hbm_budget_bytes"] = budget
+    d["mean_N"] = float(np.mean([r.N for r in reqs]))
+    log(f"[bench] {name}: {st.n_done} done in {st.makespan:.1f}s, {st.throughput:.3f} req/s, "
+        f"p50 {st.p50:.2f}s p99 {st.p99:.2f}s, max active {st.max_active}, "
+        f"CG {d['cg_converged']}/{d['cg_requests']} converged, builds late by at most {d['build_late_max_s']:.1f}s")
+    return d
 
 
 def serve_compare(reqs, slots, hbm_frac=0.8):
@@ -148,7 +210,39 @@ def serve_compare(reqs, slots, hbm_frac=0.8):
 def run_serving(args):
     """Matern-3/2 GP requests: continuous batching on the FP32-mixed path
     against serving each alone on the FP64 path.  Host construction happens
-    before the clock starts, in worker processes."""
+    before the clock starts, in worker processes; with ``--serve-legs
+    batched`` or ``alone`` one leg runs on requests built while they are
+    served (the configuration's 512 requests do not fit in host memory at
+    once)."""
+    if args.serve_legs != "both":
+        leg = {"batched": ("batched_fp32mix", "fp32mix", args.slots), "alone": ("alone_fp64", "fp64", 1)}[args.serve_legs]
+        t0 = time.perf_counter()
+        reqs, ex, th = start_stream(args.requests, args.seed, args.rate, args.cg_max_iter,
+                                    args.build_workers, args.build_window)
+        head = min(args.build_head, args.build_window, args.requests)
+        t_log = time.perf_counter()
+        while True:   # a head start: the first requests built before the clock starts
+            done = sum(1 for r in reqs[:head] if r.future is not None and r.future.done())
+            if done >= head:
+                break
+            if time.perf_counter() - t_log > 15.0:
+                t_log = time.perf_counter()
+                log(f"[bench] head start: {done}/{head} built after {t_log - t0:.0f}s")
+            time.sleep(0.2)
+        log(f"[bench] head start: {head} requests built in {time.perf_counter() - t0:.1f}s")
+        d = serve_one(reqs, leg[0], leg[1], leg[2], drop_host=True)
+        th.join()
+        ex.shutdown()
+        out = {"metric": "served GP requests/s and p50/p99 latency (Matern-3/2)", "value": d["throughput"],
+               "unit": "requests/s", "higher_is_better": True, "n_gpus": 1,
+               "dtype": "f32-mixed" if leg[1] == "fp32mix" else "f64", "data": "synthetic",
+               "config": {"workload": f"serving_matern32_{args.requests}req_N1e4-4e5_density10_reltol1e-6_leaf300"
+                                      f"_rate{args.rate}_8rhs_cg_tol1e-6_nugget1e-2_{leg[0]}",
+                          "requests": args.requests, "rate_per_s": args.rate, "slots": leg[2],
+                          "cg_max_iter": args.cg_max_iter, "build_workers": args.build_workers},
+               leg[0]: d, "vs_baseline": None}
+        print(json.dumps(out), flush=True)
+        return
     t0 = time.perf_counter()
     ex, futs = start_request_builds(args.requests, args.seed, args.rate, args.cg_max_iter, args.build_workers)
     reqs = collect_requests(ex, futs)
@@ -188,15 +282,33 @@ def run(args):
         x = np.random.default_rng(args.seed + 1).random(N) - 0.5
         xd = torch.from_numpy(x).to(dev); yd = torch.empty_like(xd)
         m = H2MatHIP(h2, mode="aot", host_blocks=True)
-        dt = _time(lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr), K, W, sync)
+        tune = m.autotune(0)
+        mv = lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr)   # noqa: E731
+        dt = _time(mv, K, W, sync)
+        # the same steps again with HIP events around every phase
+        m.set_profiling(True)
+        _time(mv, K, 0, sync)
         info = m.info()
+        m.set_profiling(False)
+        blk_bytes = 8.0 * (info["B_elems"] + info["D_elems"])
         y = yd.cpu().numpy()
         rows = np.sort(np.random.default_rng(args.seed + 2).choice(N, 1000, replace=False))
         yref = direct_nbody(C.KRNL_COULOMB, [], coord, x, rows)
         out.update({"metric": "H2 matvecs/s (Coulomb N=80k AOT, FP64)", "value": 1.0 / dt, "unit": "matvecs/s",
                     "ms_per_step": dt * 1e3, "dtype": "f64",
                     "hbm_GBps_model": info["hbm_bytes_model"] / dt / 1e9,
+                    "hbm_frac_model": info["hbm_bytes_model"] / dt / 1e9 / 8000.0,
+                    "block_variant": tune["variants"][0],
+                    "roofline": {"bound": "hbm", "kernel": "k_blocks_aot", "kernel_ms": info["t_blocks"],
+                                 "achieved": blk_bytes / (info["t_blocks"] * 1e-3) / 1e9, "peak": 8000.0,
+                                 "unit": "GB/s",
+                                 "frac": blk_bytes / (info["t_blocks"] * 1e-3) / 1e9 / 8000.0},
+                    "phases_ms": {k: info[k] for k in ("t_fwd", "t_blocks", "t_reduce", "t_bwd", "t_final",
+                                                       "t_total")},
+                    "stored_GB": blk_bytes / 1e9,
                     "rel_err_vs_direct_nbody_1000rows": _rel(y[rows], yref), "build_s": tb})
+        log(f"[bench] coulomb80k AOT: {dt * 1e3:.3f} ms per matvec, model {info['hbm_bytes_model'] / dt / 1e9:.0f} GB/s; "
+            f"block launch {info['t_blocks']:.3f} ms for {blk_bytes / 1e9:.2f} GB; phases {out['phases_ms']}")
         if not args.no_cpu:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle as O
