@@ -55,10 +55,17 @@ class Request:
     iters: int = 0
     relres: float = math.nan
     rejected: bool = False
+    # a request built while the server runs: h2 and rhs are None until
+    # ``future`` (resolving to a built Request) is done; ``loaded`` is when
+    # the server took it over (requests whose build is late enter the queue
+    # then, not at their arrival)
+    future: object = None
+    loaded: float = math.nan
+    n_mat: int = 0
 
     @property
     def N(self) -> int:
-        return self.h2.krnl_mat_size
+        return self.h2.krnl_mat_size if self.h2 is not None else self.n_mat
 
     @property
     def latency(self) -> float:
@@ -130,7 +137,7 @@ class Server:
 
     def __init__(self, hbm_budget: int, max_active: int = 64, mode: str = "jit",
                  prec: str = "fp32mix", cg_chunk: int = 8, device: str = "cuda:0",
-                 refine: bool | None = None, max_refine: int = 4):
+                 refine: bool | None = None, max_refine: int = 4, drop_host: bool = False):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
@@ -138,6 +145,7 @@ class Server:
         self.mode, self.prec = mode, prec
         self.refine = (prec == "fp32mix") if refine is None else bool(refine)
         self.max_refine = int(max_refine)
+        self.drop_host = bool(drop_host)   # free a request's host representation once uploaded
         self.cg_chunk = int(cg_chunk)
         self.dev = torch.device(device)
         # two streams at the default priority
@@ -188,7 +196,9 @@ class Server:
         torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in pending:
-            r.nbytes = request_bytes(r, self.mode, self.prec, self.refine)
+            if r.h2 is not None:
+                r.nbytes = request_bytes(r, self.mode, self.prec, self.refine)
+                r.n_mat = r.h2.krnl_mat_size
             r.iters = 0
         queue: deque[Request] = deque()
         cg: list[_Active] = []
@@ -207,8 +217,22 @@ class Server:
                 log(f"[serve] t={t:.1f}s done {st.n_done}/{len(pending)} active cg {len(cg)} "
                     f"products in flight {sum(len(g) for g, _ in mm_inflight)} queued {len(queue)} "
                     f"bytes {used / 1e9:.1f} GB")
+            late = False
             while i < len(pending) and (not realtime or pending[i].arrival <= t):
-                queue.append(pending[i])
+                r = pending[i]
+                if r.h2 is None:
+                    f = r.future
+                    if f is None or not f.done():
+                        late = True
+                        break
+                    built = f.result()
+                    built = built[0] if isinstance(built, tuple) else built
+                    r.h2, r.rhs, r.future = built.h2, built.rhs, None
+                    r.shift, r.tol, r.max_iter = built.shift, built.tol, built.max_iter
+                    r.n_mat = r.h2.krnl_mat_size
+                    r.nbytes = request_bytes(r, self.mode, self.prec, self.refine)
+                r.loaded = now()
+                queue.append(r)
                 i += 1
             # ---- admission, first come first served
             new_cg, new_mm = [], []
@@ -225,6 +249,11 @@ class Server:
                 queue.popleft()
                 act = self._admit(r, self.s_cg if r.kind == "cg" else self.s_mm)
                 r.admitted = now()
+                if self.drop_host:
+                    r.h2 = None
+                    act.mat.h2 = None
+                    if act.mat64 is not None:
+                        act.mat64.h2 = None
                 used += r.nbytes
                 st.peak_bytes = max(st.peak_bytes, used)
                 n_active += 1
@@ -326,9 +355,9 @@ class Server:
             # ---- only products in flight: wait for the oldest one
             if not cg and mm_inflight:
                 mm_inflight[0][1].synchronize()
-            # ---- idle: wait for the next arrival
+            # ---- idle: wait for the next arrival (or its build)
             elif not cg and not queue and i < len(pending) and realtime:
-                time.sleep(max(0.0, min(pending[i].arrival - now(), 0.05)))
+                time.sleep(0.005 if late else max(0.0, min(pending[i].arrival - now(), 0.05)))
         torch.cuda.synchronize()
         for a in to_free:
             a.mat.close()
