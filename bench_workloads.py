@@ -442,12 +442,21 @@ def run(args):
             tb = time.perf_counter() - t0
             x = np.random.default_rng(args.seed + 1).random(N) - 0.5
             xd = torch.from_numpy(x).to(dev); yd = torch.empty_like(xd)
-            ys, ts = {}, {}
+            ys, ts, ph, tn = {}, {}, {}, {}
             for prec in ("fp64", "fp32mix"):
                 m = H2MatHIP(h2, mode="jit", prec=prec)
-                m.autotune(0)
-                ts[prec] = _time(lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr), K, W, sync)
+                tn[prec] = m.autotune(0)
+                mv = lambda: m.matvec_dev(xd.data_ptr(), yd.data_ptr(), sptr)   # noqa: E731
+                ts[prec] = _time(mv, K, W, sync)
                 ys[prec] = yd.cpu().numpy()
+                # the same steps again with HIP events around every phase (in order on one stream)
+                m.set_profiling(True)
+                _time(mv, K, 0, sync)
+                info = m.info()
+                m.set_profiling(False)
+                ph[prec] = {k: round(info[k], 3) for k in ("t_fwd", "t_blocks", "t_reduce", "t_bwd", "t_final", "t_total")}
+                log(f"[bench] {name} 2M {prec}: {ts[prec] * 1e3:.2f} ms per matvec; profiled phases ms {ph[prec]}; "
+                    f"tune {tn[prec]}")
                 m.close()
             rows = np.sort(np.random.default_rng(args.seed + 2).choice(N, 1000, replace=False))
             yref = direct_nbody(kid, param, coord, x, rows)
@@ -456,7 +465,9 @@ def run(args):
                          "fp32mix_rel_err_vs_fp64_gpu": _rel(ys["fp32mix"], ys["fp64"]),
                          "fp64_rel_err_vs_direct_nbody_1000rows": _rel(ys["fp64"][rows], yref),
                          "fp32mix_rel_err_vs_direct_nbody_1000rows": _rel(ys["fp32mix"][rows], yref),
-                         "build_s": tb, "levels": h2.max_level + 1, "adm_pairs": h2.n_r_adm_pair}
+                         "build_s": tb, "levels": h2.max_level + 1, "adm_pairs": h2.n_r_adm_pair,
+                         "phases_ms": ph, "tune": {k: {"variant": v["variants"][0], "d_concurrent": v["d_concurrent"],
+                                                       "level_split": v["level_split"]} for k, v in tn.items()}}
             log(f"[bench] {name} 2M: fp64 {1 / ts['fp64']:.2f}/s fp32mix {1 / ts['fp32mix']:.2f}/s "
                 f"errs {res[name]['fp32mix_rel_err_vs_fp64_gpu']:.2e} {res[name]['fp64_rel_err_vs_direct_nbody_1000rows']:.2e}")
             del h2
