@@ -72,7 +72,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--check-rows", type=int, default=1000)
     ap.add_argument("--workload", default="gaussian1m",
-                    choices=["gaussian1m", "coulomb80k", "rpy_matmul", "precision2m", "serving", "periodic"],
+                    choices=["gaussian1m", "coulomb80k", "rpy_matmul", "precision2m", "serving", "periodic",
+                             "serving-replay"],
                     help="gaussian1m: the headline (default); the others are the remaining "
                          "configurations, each printed as its own JSON line")
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
@@ -80,8 +81,15 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: most requests on the GPU at once")
     ap.add_argument("--cg-max-iter", type=int, default=3000, help="serving: CG iteration cap")
     ap.add_argument("--build-workers", type=int, default=4, help="serving: host construction processes")
-    ap.add_argument("--serve-legs", default="both", choices=["both", "batched", "alone"],
-                    help="serving: both legs on prebuilt requests, or one leg on requests built while served")
+    ap.add_argument("--serve-legs", default="both", choices=["both", "batched", "alone", "alone-service"],
+                    help="serving: both legs on prebuilt requests, one leg on requests built while served, "
+                         "or the FP64-alone service time of each request (back to back, no arrival clock)")
+    ap.add_argument("--serve-first", type=int, default=0, help="serving, alone-service: first request")
+    ap.add_argument("--serve-count", type=int, default=0, help="serving, alone-service: requests (0: the rest)")
+    ap.add_argument("--replay-files", nargs="*", default=[],
+                    help="serving-replay: JSON outputs of --serve-legs alone-service covering every request")
+    ap.add_argument("--replay-rates", type=float, nargs="*", default=[0.3, 0.6],
+                    help="serving-replay: arrival rates to replay")
     ap.add_argument("--build-window", type=int, default=64,
                     help="serving, one leg: most requests built but not yet admitted")
     ap.add_argument("--build-head", type=int, default=16,

@@ -68,6 +68,7 @@ def start_request_builds(n, seed, rate, cg_max_iter, workers):
     from concurrent.futures import ProcessPoolExecutor
     from h2pack_amd.serve import request_specs
     specs = request_specs(n, seed, 10_000, 400_000, rate)
+    specs = specs[first:] if count is None else specs[first:first + count]
     kw = {"max_iter": cg_max_iter}
     workers = max(1, workers)
     threads = max(1, 16 // workers)
@@ -87,17 +88,19 @@ def start_request_builds(n, seed, rate, cg_max_iter, workers):
     return ex, futs
 
 
-def start_stream(n, seed, rate, cg_max_iter, workers, window):
+def start_stream(n, seed, rate, cg_max_iter, workers, window, first=0, count=None):
     """Requests built while they are served: placeholders in arrival order,
     each given a build future by a feeder thread that keeps at most
     ``window`` requests built or building but not yet admitted by the server
-    (so host memory stays bounded).  Call before the GPU is first used."""
+    (so host memory stays bounded); requests ``first`` .. ``first + count``
+    of the ``n``-request sequence.  Call before the GPU is first used."""
     import math as _m
     import multiprocessing as mp
     import threading
     from concurrent.futures import ProcessPoolExecutor
     from h2pack_amd.serve import Request, request_specs
     specs = request_specs(n, seed, 10_000, 400_000, rate)
+    specs = specs[first:] if count is None else specs[first:first + count]
     kw = {"max_iter": cg_max_iter}
     workers = max(1, workers)
     threads = max(1, 16 // workers)
@@ -207,6 +210,79 @@ def serve_compare(reqs, slots, hbm_frac=0.8):
             "mean_N": float(np.mean([r.N for r in reqs]))}
 
 
+def run_alone_service(args):
+    """Service times of the FP64-alone server: requests ``--serve-first`` ..
+    ``+ --serve-count`` of the ``--requests`` sequence served one at a time,
+    back to back (no arrival clock), each timed from the start of its upload
+    to its result.  The arrival times of a rate only scale the same draws,
+    so these times give the FIFO latencies at every rate (serving-replay)."""
+    count = args.serve_count if args.serve_count > 0 else args.requests - args.serve_first
+    reqs, ex, th = start_stream(args.requests, args.seed, args.rate, args.cg_max_iter, args.build_workers,
+                                args.build_window, first=args.serve_first, count=count)
+    head = min(args.build_head, args.build_window, count)
+    t0 = time.perf_counter()
+    while sum(1 for r in reqs[:head] if r.future is not None and r.future.done()) < head:
+        time.sleep(0.2)
+    log(f"[bench] head start: {head} requests built in {time.perf_counter() - t0:.1f}s")
+    import torch
+    from h2pack_amd.serve import Server
+    free, _ = torch.cuda.mem_get_info()
+    srv = Server(hbm_budget=int(0.8 * free), max_active=1, prec="fp64", cg_chunk=8, drop_host=True)
+    log(f"[bench] FP64 alone, back to back: requests {args.serve_first}..{args.serve_first + count - 1}")
+    st = srv.run(reqs, realtime=False, log=log)
+    srv.close()
+    th.join()
+    ex.shutdown()
+    rows = [{"rid": r.rid, "kind": r.kind, "N": r.N, "service_s": r.finished - r.admit_start,
+             "iters": r.iters, "relres": r.relres}
+            for r in sorted(reqs, key=lambda q: q.rid)]
+    cg = [x for x in rows if x["kind"] == "cg"]
+    out = {"metric": "FP64-alone service times (Matern-3/2 GP requests)", "unit": "s",
+           "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"serving_matern32_{args.requests}req_N1e4-4e5_density10_reltol1e-6_leaf300"
+                                  f"_8rhs_cg_tol1e-6_nugget1e-2_alone_fp64_service",
+                      "requests": args.requests, "first": args.serve_first, "count": count,
+                      "cg_max_iter": args.cg_max_iter, "seed": args.seed},
+           "n_done": st.n_done, "sum_service_s": float(sum(x["service_s"] for x in rows)),
+           "cg_converged": sum(1 for x in cg if x["relres"] <= 1e-6), "cg_requests": len(cg),
+           "requests_detail": rows}
+    log(f"[bench] FP64 alone: {st.n_done} requests, service {out['sum_service_s']:.1f}s in total, "
+        f"CG {out['cg_converged']}/{len(cg)} converged")
+    print(json.dumps(out), flush=True)
+
+
+def run_serving_replay(args):
+    """FP64-alone latencies at each of ``--replay-rates`` from measured
+    service times (the JSON lines of run_alone_service covering every
+    request once), by the FIFO recurrence; no GPU."""
+    from h2pack_amd.serve import fifo_latencies, request_specs
+    rows = {}
+    for f in args.replay_files:
+        with open(f) as fh:
+            line = [ln for ln in fh.read().splitlines() if ln.startswith("{")][-1]
+        for x in json.loads(line)["requests_detail"]:
+            rows[x["rid"]] = x
+    n = args.requests
+    missing = [k for k in range(n) if k not in rows]
+    if missing:
+        raise SystemExit(f"service times missing for {len(missing)} requests (first {missing[:5]})")
+    svc = np.array([rows[k]["service_s"] for k in range(n)])
+    res = {"sum_service_s": float(svc.sum()), "capacity_req_per_s": n / float(svc.sum())}
+    for rate in args.replay_rates:
+        arr = np.array([s["arrival"] for s in request_specs(n, args.seed, 10_000, 400_000, rate)])
+        lat = fifo_latencies(arr, svc)
+        ends = arr + lat
+        res[f"rate_{rate}"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                               "mean": float(lat.mean()), "makespan": float(ends.max()),
+                               "throughput": n / float(ends.max()), "last_arrival": float(arr[-1])}
+        log(f"[bench] FP64 alone at {rate}/s (FIFO replay): {n / ends.max():.3f} req/s, "
+            f"p50 {np.percentile(lat, 50):.1f}s p99 {np.percentile(lat, 99):.1f}s")
+    print(json.dumps({"metric": "FP64-alone served latency by FIFO replay of measured service times",
+                      "unit": "s", "n_gpus": 1, "dtype": "f64",
+                      "config": {"workload": f"serving_matern32_{n}req_alone_fp64_replay", "seed": args.seed},
+                      **res}), flush=True)
+
+
 def run_serving(args):
     """Matern-3/2 GP requests: continuous batching on the FP32-mixed path
     against serving each alone on the FP64 path.  Host construction happens
@@ -261,6 +337,10 @@ def run_serving(args):
 
 
 def run(args):
+    if args.workload == "serving-replay":
+        return run_serving_replay(args)
+    if args.workload == "serving" and args.serve_legs == "alone-service":
+        return run_alone_service(args)
     if args.workload == "serving":
         return run_serving(args)
     import torch

@@ -48,13 +48,30 @@ __global__ void k_gather(const double *__restrict__ x, const int *__restrict__ i
 }
 
 // Fixed-order segmented reduction of the block partials into y1.
+// v + arena[contrib[k] + e] for k = kb, kb + step, ... < ke, added in list
+// order.  The offsets and values of H2P_GATHER_AHEAD entries are loaded
+// before their adds, so that many loads are in flight instead of one
+// offset-then-value chain per entry; the sum is the plain loop's, bit for bit.
+#define H2P_GATHER_AHEAD 8
+__device__ __forceinline__ double ordered_gather_sum(double v, const long long *__restrict__ contrib,
+                                                     const double *__restrict__ arena, int kb, int ke, int step,
+                                                     int e) {
+    int k = kb;
+    for (; k + (H2P_GATHER_AHEAD - 1) * step < ke; k += H2P_GATHER_AHEAD * step) {
+        double a[H2P_GATHER_AHEAD];
+#pragma unroll
+        for (int u = 0; u < H2P_GATHER_AHEAD; u++) a[u] = arena[contrib[k + u * step] + e];
+#pragma unroll
+        for (int u = 0; u < H2P_GATHER_AHEAD; u++) v += a[u];
+    }
+    for (; k < ke; k += step) v += arena[contrib[k] + e];
+    return v;
+}
+
 __device__ __forceinline__ void reduce_y1_body(const RedTask &t, const long long *__restrict__ contrib,
                                                const double *__restrict__ arena, double *__restrict__ y1) {
-    for (int e = threadIdx.x; e < t.len; e += blockDim.x) {
-        double s = 0.0;
-        for (int k = t.cbeg; k < t.cend; k++) s += arena[contrib[k] + e];
-        y1[t.dst + e] = s;
-    }
+    for (int e = threadIdx.x; e < t.len; e += blockDim.x)
+        y1[t.dst + e] = ordered_gather_sum(0.0, contrib, arena, t.cbeg, t.cend, 1, e);
 }
 
 __global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ tasks,
@@ -69,58 +86,71 @@ __global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ t
 //
 // Upward sweep (src/H2Pack_matvec.c:95-177): one workgroup per node and
 // 256-column chunk of its U (so a node with a large basis spreads over
-// several workgroups); lanes own the chunk's columns (4 strips), the four
-// waves take contiguous quarters of the rows and keep 4 rows x 4 strips of
-// loads in flight; the four wave partials are added in LDS in a fixed order.
-__device__ __forceinline__ void fwd2_body(const FwdTask &t, const double *__restrict__ U,
-                                          double *__restrict__ xin) {
-    __shared__ double red[4][4][H2P_WAVE];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// several workgroups); lanes own the chunk's columns (NS strips of 64, only
+// the strips that hold columns), the four waves take contiguous quarters of
+// the rows and load 8 rows x NS strips before their FMAs (rows in order, so
+// each column's sum runs over the rows in order whatever the unrolling); the
+// four wave partials are added in LDS in a fixed order.
+template <int NS>
+__device__ __forceinline__ void fwd2_strips(const FwdTask &t, const double *__restrict__ U,
+                                            double *__restrict__ xin, double (*red)[4][H2P_WAVE]) {
+    // the wave index is the same on every lane: said so, the row addresses
+    // stay scalar and each load needs only its column offset per lane
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double *Ut = U + t.u_off;
     const double *in = xin + t.in_off;
     const int q = (t.nrow + 3) >> 2;
     const int i0 = w * q, i1 = min(t.nrow, i0 + q);
-    {
-        const int cs = t.col0;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        int cc[4];
-        bool ok[4];
+    const int cs = t.col0;
+    double acc[NS];
+    int cc[NS];
+    bool ok[NS];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            cc[k] = cs + k * H2P_WAVE + lane;
-            ok[k] = cc[k] < t.r;
-            cc[k] = ok[k] ? cc[k] : 0;
-        }
-        int i = i0;
-        for (; i + 3 < i1; i += 4) {
-            double u[4][4];
+    for (int k = 0; k < NS; k++) {
+        acc[k] = 0.0;
+        cc[k] = cs + k * H2P_WAVE + lane;
+        ok[k] = cc[k] < t.r;
+        cc[k] = ok[k] ? cc[k] : 0;
+    }
+    int i = i0;
+    for (; i + 7 < i1; i += 8) {
+        double u[8][NS];
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+        for (int a = 0; a < 8; a++)
 #pragma unroll
-                for (int k = 0; k < 4; k++) u[a][k] = Ut[(long long)(i + a) * t.r + cc[k]];
+            for (int k = 0; k < NS; k++) u[a][k] = Ut[(long long)(i + a) * t.r + cc[k]];
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const double v = in[i + a];
+        for (int a = 0; a < 8; a++) {
+            const double v = in[i + a];
 #pragma unroll
-                for (int k = 0; k < 4; k++) acc[k] += u[a][k] * v;
-            }
-        }
-        for (; i < i1; i++) {
-            const double v = in[i];
-#pragma unroll
-            for (int k = 0; k < 4; k++) acc[k] += Ut[(long long)i * t.r + cc[k]] * v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; k++) red[k][w][lane] = acc[k];
-        __syncthreads();
-        if (w == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (ok[k])
-                    xin[t.out_off + cc[k]] = (red[k][0][lane] + red[k][1][lane]) + (red[k][2][lane] + red[k][3][lane]);
+            for (int k = 0; k < NS; k++) acc[k] += u[a][k] * v;
         }
     }
+    for (; i < i1; i++) {
+        const double v = in[i];
+#pragma unroll
+        for (int k = 0; k < NS; k++) acc[k] += Ut[(long long)i * t.r + cc[k]] * v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NS; k++) red[k][w][lane] = acc[k];
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int k = 0; k < NS; k++)
+            if (ok[k])
+                xin[t.out_off + cc[k]] = (red[k][0][lane] + red[k][1][lane]) + (red[k][2][lane] + red[k][3][lane]);
+    }
+}
+
+__device__ __forceinline__ void fwd2_body(const FwdTask &t, const double *__restrict__ U,
+                                          double *__restrict__ xin) {
+    __shared__ double red[4][4][H2P_WAVE];
+    const int ns = (min(t.r - t.col0, 4 * H2P_WAVE) + H2P_WAVE - 1) / H2P_WAVE;
+    if (ns <= 1) fwd2_strips<1>(t, U, xin, red);
+    else if (ns == 2) fwd2_strips<2>(t, U, xin, red);
+    else if (ns == 3) fwd2_strips<3>(t, U, xin, red);
+    else fwd2_strips<4>(t, U, xin, red);
 }
 
 __global__ void __launch_bounds__(256) k_fwd2(const FwdTask *__restrict__ tasks,
@@ -137,6 +167,35 @@ struct BwdTask2 {
     int nrow, r, y1_off, push, out_off, row0;
 };
 #define H2P_MAXR 2048
+// sum_c UT[c * ld] * yv[c] for one row of U (U^T stored row-major, so the
+// lanes' rows are contiguous for each c): four accumulators taking every
+// fourth c, 16 values loaded before their FMAs; each accumulator's FMAs run
+// over its c in order, so the sum is the same for any unrolling.
+__device__ __forceinline__ double ut_row_dot(const double *__restrict__ ut, int ld, int r, const double *yv) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int c = 0;
+    for (; c + 15 < r; c += 16) {
+        double u[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) u[j] = ut[(long long)(c + j) * ld];
+#pragma unroll
+        for (int j = 0; j < 16; j += 4) {
+            a0 += u[j] * yv[c + j];
+            a1 += u[j + 1] * yv[c + j + 1];
+            a2 += u[j + 2] * yv[c + j + 2];
+            a3 += u[j + 3] * yv[c + j + 3];
+        }
+    }
+    for (; c + 3 < r; c += 4) {
+        a0 += ut[(long long)c * ld] * yv[c];
+        a1 += ut[(long long)(c + 1) * ld] * yv[c + 1];
+        a2 += ut[(long long)(c + 2) * ld] * yv[c + 2];
+        a3 += ut[(long long)(c + 3) * ld] * yv[c + 3];
+    }
+    for (; c < r; c++) a0 += ut[(long long)c * ld] * yv[c];
+    return (a0 + a1) + (a2 + a3);
+}
+
 __device__ __forceinline__ void bwd2_body(const BwdTask2 &t, const double *__restrict__ UT,
                                           const double *__restrict__ y1, double *__restrict__ yp) {
     __shared__ double yv[H2P_MAXR];
@@ -145,17 +204,7 @@ __device__ __forceinline__ void bwd2_body(const BwdTask2 &t, const double *__res
     __syncthreads();
     const int row = t.row0 + threadIdx.x;
     if (row >= t.nrow) return;
-    const double *ut = UT + t.ut_off + row;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int c = 0;
-    for (; c + 3 < t.r; c += 4) {
-        a0 += ut[(long long)c * t.nrow] * yv[c];
-        a1 += ut[(long long)(c + 1) * t.nrow] * yv[c + 1];
-        a2 += ut[(long long)(c + 2) * t.nrow] * yv[c + 2];
-        a3 += ut[(long long)(c + 3) * t.nrow] * yv[c + 3];
-    }
-    for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
-    yp[t.out_off + row] = (a0 + a1) + (a2 + a3);
+    yp[t.out_off + row] = ut_row_dot(UT + t.ut_off + row, t.nrow, t.r, yv);
 }
 
 __global__ void __launch_bounds__(256) k_bwd2(const BwdTask2 *__restrict__ tasks,
@@ -191,19 +240,12 @@ __device__ __forceinline__ void leaf_stage_y1(const LeafTask2 &t, const long lon
                                               double *yv, double *part) {
     const int G = t.r > 0 ? min(8, (int)blockDim.x / t.r) : 1;
     if (G <= 1) {
-        for (int c = threadIdx.x; c < t.r; c += blockDim.x) {
-            double v = t.push ? yp[t.s_off + c] : 0.0;
-            for (int k = t.cb1; k < t.ce1; k++) v += arena[contrib[k] + c];
-            yv[c] = v;
-        }
+        for (int c = threadIdx.x; c < t.r; c += blockDim.x)
+            yv[c] = ordered_gather_sum(t.push ? yp[t.s_off + c] : 0.0, contrib, arena, t.cb1, t.ce1, 1, c);
         return;
     }
     const int g = threadIdx.x / t.r, c = threadIdx.x - g * t.r;
-    if (g < G) {
-        double v = 0.0;
-        for (int k = t.cb1 + g; k < t.ce1; k += G) v += arena[contrib[k] + c];
-        part[g * t.r + c] = v;
-    }
+    if (g < G) part[g * t.r + c] = ordered_gather_sum(0.0, contrib, arena, t.cb1 + g, t.ce1, G, c);
     __syncthreads();
     for (int c2 = threadIdx.x; c2 < t.r; c2 += blockDim.x) {
         double v = t.push ? yp[t.s_off + c2] : 0.0;
@@ -214,23 +256,13 @@ __device__ __forceinline__ void leaf_stage_y1(const LeafTask2 &t, const long lon
 
 __device__ __forceinline__ double leaf_u_row(const LeafTask2 &t, const double *__restrict__ UT, const double *yv,
                                              int i) {
-    const double *ut = UT + t.ut_off + i;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int c = 0;
-    for (; c + 3 < t.r; c += 4) {
-        a0 += ut[(long long)c * t.nrow] * yv[c];
-        a1 += ut[(long long)(c + 1) * t.nrow] * yv[c + 1];
-        a2 += ut[(long long)(c + 2) * t.nrow] * yv[c + 2];
-        a3 += ut[(long long)(c + 3) * t.nrow] * yv[c + 3];
-    }
-    for (; c < t.r; c++) a0 += ut[(long long)c * t.nrow] * yv[c];
-    return (a0 + a1) + (a2 + a3);
+    return ut_row_dot(UT + t.ut_off + i, t.nrow, t.r, yv);
 }
 
 __device__ __forceinline__ double leaf_d_row(const LeafTask2 &t, const long long *__restrict__ contrib,
                                              const double *__restrict__ arena, const double *__restrict__ px,
                                              double shift, int i, double v) {
-    for (int k = t.cb0; k < t.ce0; k++) v += arena[contrib[k] + i];
+    v = ordered_gather_sum(v, contrib, arena, t.cb0, t.ce0, 1, i);
     if (shift != 0.0) v += shift * px[t.vec_s + i];
     return v;
 }

@@ -62,6 +62,9 @@ class Request:
     future: object = None
     loaded: float = math.nan
     n_mat: int = 0
+    # when its admission (upload) began: with one request at a time,
+    # finished - admit_start is the request's whole service time
+    admit_start: float = math.nan
 
     @property
     def N(self) -> int:
@@ -247,6 +250,7 @@ class Server:
                 if used + r.nbytes > self.budget:
                     break
                 queue.popleft()
+                r.admit_start = now()
                 act = self._admit(r, self.s_cg if r.kind == "cg" else self.s_mm)
                 r.admitted = now()
                 if self.drop_host:
@@ -372,6 +376,27 @@ class Server:
             st.mean = float(np.mean(lat))
         st.throughput = st.n_done / st.makespan if st.makespan > 0 else 0.0
         return st
+
+
+def fifo_latencies(arrivals, services) -> np.ndarray:
+    """Latencies of a single first-come-first-served server: request k starts
+    at max(its arrival, the end of request k - 1) and takes services[k]
+    seconds.  This is exactly the one-request-at-a-time server
+    (``max_active`` = 1) when each request's service time does not depend on
+    the others, so its latencies at any arrival rate follow from service
+    times measured once, back to back."""
+    arr = np.asarray(arrivals, dtype=np.float64)
+    svc = np.asarray(services, dtype=np.float64)
+    if arr.shape != svc.shape:
+        raise ValueError("one service time per arrival")
+    if np.any(np.diff(arr) < 0) or np.any(svc < 0):
+        raise ValueError("arrivals must be sorted and service times non-negative")
+    lat = np.empty_like(arr)
+    end = -np.inf
+    for k in range(arr.size):
+        end = max(arr[k], end) + svc[k]
+        lat[k] = end - arr[k]
+    return lat
 
 
 def request_specs(n: int, seed: int, n_min: int = 10_000, n_max: int = 400_000,
