@@ -50,6 +50,13 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def _host_threads():
+    """Host threads the builds may use: the GPU box's share is 16 CPUs,
+    whatever the machine shows; fewer where fewer exist (BLAS given more
+    threads than CPUs has crashed in the workers)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def _build_worker(spec, kw, threads):
     os.environ["H2P_BUILD_THREADS"] = str(threads)
     from threadpoolctl import threadpool_limits
@@ -68,10 +75,9 @@ def start_request_builds(n, seed, rate, cg_max_iter, workers):
     from concurrent.futures import ProcessPoolExecutor
     from h2pack_amd.serve import request_specs
     specs = request_specs(n, seed, 10_000, 400_000, rate)
-    specs = specs[first:] if count is None else specs[first:first + count]
     kw = {"max_iter": cg_max_iter}
     workers = max(1, workers)
-    threads = max(1, 16 // workers)
+    threads = max(1, _host_threads() // workers)
     # the workers inherit these at start: bounded BLAS / OpenMP threads that
     # sleep instead of spinning when idle, so the processes do not starve
     # each other (restored in this process once they have started)
@@ -103,7 +109,7 @@ def start_stream(n, seed, rate, cg_max_iter, workers, window, first=0, count=Non
     specs = specs[first:] if count is None else specs[first:first + count]
     kw = {"max_iter": cg_max_iter}
     workers = max(1, workers)
-    threads = max(1, 16 // workers)
+    threads = max(1, _host_threads() // workers)
     # the workers start at the feeder's first submit and inherit these then
     # (bounded BLAS / OpenMP threads that sleep when idle), so they stay set
     # in this process, which builds nothing itself
@@ -158,7 +164,8 @@ def serve_one(reqs, name, prec, slots, hbm_frac=0.8, drop_host=False):
     d["mean_N"] = float(np.mean([r.N for r in reqs]))
     log(f"[bench] {name}: {st.n_done} done in {st.makespan:.1f}s, {st.throughput:.3f} req/s, "
         f"p50 {st.p50:.2f}s p99 {st.p99:.2f}s, max active {st.max_active}, "
-        f"CG {d['cg_converged']}/{d['cg_requests']} converged, builds late by at most {d['build_late_max_s']:.1f}s")
+        f"CG {d['cg_converged']}/{d['cg_requests']} converged ({d['cg_mean_iters']:.0f} iterations mean, "
+        f"{st.cg_replacements} FP64 residual replacements), builds late by at most {d['build_late_max_s']:.1f}s")
     return d
 
 

@@ -16,7 +16,7 @@ EXPORTS = ("h2p_hip_upload", "h2p_hip_matvec", "h2p_hip_matvec_dev", "h2p_hip_ma
            "h2p_hip_footprint", "h2p_hip_batch_create", "h2p_hip_batch_free",
            "h2p_hip_matvec_batch", "h2p_hip_matmul_batch", "h2p_hip_cg_start",
            "h2p_hip_cg_step", "h2p_hip_cg_poll", "h2p_hip_autotune", "h2p_hip_mm_variant",
-           "h2p_hip_matmul_dev_noalloc")
+           "h2p_hip_matmul_dev_noalloc", "h2p_hip_cg_set_replace", "h2p_hip_cg_replace")
 MM_VARIANTS = 4
 NOALLOC_BYTES = 256 << 20
 
@@ -109,6 +109,10 @@ def bind(lib: ct.CDLL) -> ct.CDLL:
     lib.h2p_hip_cg_step.restype = ct.c_int
     lib.h2p_hip_cg_poll.argtypes = [_vp, ct.c_int, _vp, _vp, _vp, _vp, _vp]
     lib.h2p_hip_cg_poll.restype = ct.c_int
+    lib.h2p_hip_cg_set_replace.argtypes = [_vp, ct.c_double]
+    lib.h2p_hip_cg_set_replace.restype = ct.c_int
+    lib.h2p_hip_cg_replace.argtypes = [_vp, ct.c_int, _vp, _vp, _vp]
+    lib.h2p_hip_cg_replace.restype = ct.c_int
     lib.h2p_hip_autotune.argtypes = [_vp, ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
     lib.h2p_hip_autotune.restype = ct.c_int
     lib.h2p_hip_mm_variant.argtypes = [_vp, ct.c_int, ct.c_int, ct.c_int]

@@ -195,6 +195,8 @@ __global__ void __launch_bounds__(256) k_mm_scatter_b(const BatchItem *__restric
 
 // ---------------------------------------------------------------------------
 // Batched CG.  Vectors are in the caller's (original) point order.
+// done: 0 iterating, 1 finished (||r|| <= tol ||b|| or max_iter), 2 paused
+// for a residual replacement (see k_cg_finish).
 struct CGScal {
     double rr;        // ||r||^2 of the current residual
     double rr_prev;
@@ -202,6 +204,8 @@ struct CGScal {
     double ps;        // p . s
     double stop;      // tol ||b||
     double bb;        // ||b||^2
+    double rr_true;   // ||r||^2 when r was last formed as b - A x
+    double replace;   // pause when rr < replace * rr_true (0: never)
     int iter, done, max_iter, pad;
 };
 
@@ -211,6 +215,7 @@ struct CGCtx {
     double *part;     // two partial sums per workgroup of this request
     CGScal *sc;
     double tol;
+    double replace;   // squared residual-drop ratio that asks for a replacement (0: never)
     int N, nblk, max_iter, pad;
 };
 
@@ -279,6 +284,8 @@ __global__ void k_cg_init_finish(int n, const CGCtx *__restrict__ cgs) {
     sc->alpha = 0.0;
     sc->ps = 0.0;
     sc->bb = bb;
+    sc->rr_true = rr;
+    sc->replace = c.replace;
     // stop when ||r|| <= tol ||b||, compared on the norms as pcg.c:47-62 does
     sc->stop = c.tol * sqrt(bb);
     sc->iter = 0;
@@ -349,7 +356,12 @@ __global__ void __launch_bounds__(256) k_cg_update(const int *__restrict__ first
     if (threadIdx.x == 0) c.part[b] = v;
 }
 
-// rr update, iteration count, stop test (pcg.c:78, 121-124)
+// rr update, iteration count, stop test (pcg.c:78, 121-124).  A solve with
+// a replacement ratio pauses (done = 2) instead of finishing, and also once
+// its recurrence residual has dropped by that ratio since r was last formed
+// as b - A x: the caller then replaces r (k_cg_replace_finish) and the solve
+// goes on with the same search direction.  Where it pauses depends only on
+// its own iterates, not on how often the caller looks.
 __global__ void k_cg_finish(int n, const CGCtx *__restrict__ cgs) {
     if ((int)blockIdx.x >= n) return;   // one workgroup per request
     const CGCtx &c = cgs[blockIdx.x];
@@ -361,7 +373,30 @@ __global__ void k_cg_finish(int n, const CGCtx *__restrict__ cgs) {
     sc->rr_prev = sc->rr;
     sc->rr = rr;
     sc->iter += 1;
-    if (!(sqrt(rr) > sc->stop) || sc->iter >= sc->max_iter) sc->done = 1;
+    const bool converged = !(sqrt(rr) > sc->stop);
+    if (sc->replace > 0.0 && sc->iter < sc->max_iter && (converged || rr < sc->replace * sc->rr_true))
+        sc->done = 2;
+    else if (converged || sc->iter >= sc->max_iter)
+        sc->done = 1;
+}
+
+// Residual replacement (the "reliable updates" of G. L. G. Sleijpen and
+// H. A. van der Vorst, Computing 56, 1996): after r = b - s with s = A x +
+// shift x formed by a more accurate product (k_cg_init's first half), rr and
+// the stop test are redone on it; rr_prev stays, so the next search
+// direction is r + (rr / rr_prev) p as usual.  Run on a running solve it
+// replaces r mid-course just the same.
+__global__ void k_cg_replace_finish(int n, const CGCtx *__restrict__ cgs) {
+    if ((int)blockIdx.x >= n) return;   // one workgroup per request
+    const CGCtx &c = cgs[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    CGScal *sc = c.sc;
+    double rr = 0.0;
+    for (int b = 0; b < c.nblk; b++) rr += c.part[2 * b];
+    sc->rr = rr;
+    sc->rr_true = rr;
+    // a finished solve (max_iter) stays finished, now reporting this residual
+    if (sc->done != 1) sc->done = (!(sqrt(rr) > sc->stop) || sc->iter >= sc->max_iter) ? 1 : 0;
 }
 
 }  // namespace h2p

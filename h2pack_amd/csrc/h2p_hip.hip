@@ -191,6 +191,7 @@ struct h2p_hip_handle {
     double *cg_x = nullptr;
     const double *cg_b = nullptr;
     double cg_shift = 0.0, cg_tol = 0.0;
+    double cg_replace = 0.0;        // squared residual-drop ratio of the next solve (h2p_hip_cg_set_replace)
     int cg_max_iter = 0, cg_nblk = 0, cg_started = 0;
 };
 
@@ -2345,6 +2346,7 @@ static CGPlan plan_cg(TableBuf &tb, int n, h2p_hip_handle *const *hs) {
         c.part = h->cg_part;
         c.sc = h->cg_sc;
         c.tol = h->cg_tol;
+        c.replace = h->cg_replace;
         c.N = h->N;
         c.nblk = h->cg_nblk;
         c.max_iter = h->cg_max_iter;
@@ -2700,6 +2702,47 @@ int h2p_hip_cg_step(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int n_it
         hipLaunchKernelGGL(k_cg_update, dim3(C.grid), dim3(256), 0, s, first, C.n, cgs);
         hipLaunchKernelGGL(k_cg_finish, dim3(C.n), dim3(64), 0, s, C.n, cgs);
     }
+    return release_slot(B, s);
+}
+
+int h2p_hip_cg_set_replace(h2p_hip_handle *h, double ratio) {
+    if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
+    if (!(ratio >= 0.0 && ratio < 1.0)) return fail(H2P_HIP_EINVAL, "the replacement ratio must be in [0, 1)");
+    h->cg_replace = ratio * ratio;
+    return 0;
+}
+
+int h2p_hip_cg_replace(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, h2p_hip_handle *const *hs_acc,
+                       void *stream) {
+    if (!B) return fail(H2P_HIP_EINVAL, "NULL batch");
+    int rc = check_handles(n, hs);
+    if (rc) return rc;
+    if ((rc = check_handles(n, hs_acc))) return rc;
+    if (n == 0) return 0;
+    for (int i = 0; i < n; i++) {
+        if (!hs[i]->cg_started) return fail(H2P_HIP_EINVAL, "CG replacement on a handle without h2p_hip_cg_start");
+        if (hs_acc[i]->N != hs[i]->N) return fail(H2P_HIP_EINVAL, "the product's handle has another order");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    TableBuf tb;
+    std::vector<const double *> xs(n);
+    std::vector<double *> ys(n);
+    std::vector<double> sh(n);
+    for (int i = 0; i < n; i++) {
+        xs[i] = hs[i]->cg_x;
+        ys[i] = hs[i]->cg_s;
+        sh[i] = hs[i]->cg_shift;
+    }
+    // s = (A + shift) x by the accurate handles, then r = b - s and the stop test
+    const MVPlan P = plan_matvec(tb, n, hs_acc, xs.data(), ys.data(), sh.data());
+    const CGPlan C = plan_cg(tb, n, hs);
+    const char *dbase = nullptr;
+    if ((rc = stage_tables(B, tb, s, &dbase))) return rc;
+    const int *first = (const int *)(dbase + C.first_off);
+    const CGCtx *cgs = (const CGCtx *)(dbase + C.cgs_off);
+    launch_matvec_b(P, dbase, s);
+    hipLaunchKernelGGL(k_cg_init, dim3(C.grid), dim3(256), 0, s, first, C.n, cgs);
+    hipLaunchKernelGGL(k_cg_replace_finish, dim3(C.n), dim3(64), 0, s, C.n, cgs);
     return release_slot(B, s);
 }
 

@@ -337,7 +337,9 @@ __device__ __forceinline__ void blk_aot_pass(const BlkTask &t, int cbase, int nc
     // column sums are FP32 over 16 rows and folded into FP64 registers.
     // Each wave takes four consecutive rows per iteration (4 x NS loads in
     // flight per lane) and reduces the four row sums with one butterfly.
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // The wave index is the same on every lane: said so, the row addresses
+    // stay scalar and each load needs only its column offset per lane.
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     TS x0[NS], acc1[NS];
     double acc1d[NS];
     int cc[NS];

@@ -197,6 +197,11 @@ class H2MatHIP:
     def set_profiling(self, on: bool):
         _check(self.lib.h2p_hip_set_profiling(self.handle, 1 if on else 0))
 
+    def set_cg_replace(self, ratio: float):
+        """Residual-replacement ratio of the next CG solve started on this
+        handle (h2p_hip_cg_set_replace; 0 never pauses)."""
+        _check(self.lib.h2p_hip_cg_set_replace(self.handle, float(ratio)))
+
     def info(self) -> dict:
         I = abi.Info()
         _check(self.lib.h2p_hip_get_info(self.handle, ct.byref(I)))
@@ -295,12 +300,20 @@ class H2Batch:
         _check(self.lib.h2p_hip_cg_step(self.handle, len(mats), self._hs(mats), int(n_iter),
                                         ct.c_void_p(stream)))
 
+    def cg_replace(self, mats, accurate, stream: int = 0):
+        """Residual replacement (h2p_hip_cg_replace): r = b - (A' + shift) x
+        for each solve of ``mats``, the product by the matching handle of
+        ``accurate`` (for instance the FP64 handle of the same matrix)."""
+        _check(self.lib.h2p_hip_cg_replace(self.handle, len(mats), self._hs(mats), self._hs(accurate),
+                                           ct.c_void_p(stream)))
+
     def cg_poll(self, mats, stream: int = 0):
-        """Waits for ``stream``; returns (done, iters, relres) lists."""
+        """Waits for ``stream``; returns (done, iters, relres) lists, done 0
+        running, 1 finished, 2 paused for a residual replacement."""
         n = len(mats)
         done = (ct.c_int * max(n, 1))()
         it = (ct.c_int * max(n, 1))()
         rr = (ct.c_double * max(n, 1))()
         _check(self.lib.h2p_hip_cg_poll(self.handle, n, self._hs(mats), done, it, rr,
                                         ct.c_void_p(stream)))
-        return [bool(done[i]) for i in range(n)], [it[i] for i in range(n)], [rr[i] for i in range(n)]
+        return [int(done[i]) for i in range(n)], [it[i] for i in range(n)], [rr[i] for i in range(n)]

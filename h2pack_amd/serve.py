@@ -77,12 +77,12 @@ class Request:
 
 def request_bytes(req: Request, mode: str, prec: str, refine: bool = False) -> int:
     """Device bytes the request holds while active (with ``refine``, a CG
-    request also holds the FP64 handle and vectors of its refinement)."""
+    request also holds the FP64 handle of its residual replacements)."""
     n_vec = req.rhs.shape[1] if req.kind == "matmul" else 0
     b = footprint(req.h2, mode, prec, n_vec=n_vec, with_cg=req.kind == "cg")
     vec = 2 * req.rhs.size * 8            # b and x, or X and Y
     if refine and req.kind == "cg":
-        b += footprint(req.h2, mode, "fp64") + 3 * req.rhs.size * 8   # A64, and r, d, y
+        b += footprint(req.h2, mode, "fp64")
     return b + vec
 
 
@@ -92,13 +92,9 @@ class _Active:
     mat: H2MatHIP
     a: object                  # device b / X
     b: object                  # device x / Y
-    # iterative refinement of an FP32-mixed CG solve (see Server)
+    # FP64 handle of an FP32-mixed CG solve's residual replacements (see Server)
     mat64: H2MatHIP | None = None
-    r: object = None           # FP64 residual b - (A64 + shift) x, the rhs of a correction
-    d: object = None           # correction solve's iterate
-    y: object = None           # (A64 + shift) x
-    correcting: bool = False
-    refines: int = 0
+    refines: int = 0           # residual replacements so far
 
 
 @dataclass
@@ -113,6 +109,7 @@ class ServeStats:
     max_active: int = 0
     peak_bytes: int = 0
     cg_iters: int = 0
+    cg_replacements: int = 0   # FP64 residual replacements of FP32-mixed solves
     ticks: int = 0
     latencies: list = field(default_factory=list)
 
@@ -125,29 +122,33 @@ class ServeStats:
 class Server:
     """Continuous batching of GP requests on one GPU.
 
-    CG requests on the FP32-mixed path are refined (``refine``, default on
-    for FP32-mixed): the FP32 operator's rounding is relative to |A| |x|,
-    so on a system with condition number kappa the residual its CG reaches
-    is the FP64 operator's only to ~1e-7 kappa.  After the FP32 solve the
-    server computes the residual r = b - (A64 + shift) x with an FP64 handle
-    of the same representation; while |r| > tol |b| it solves the correction
-    (A32 + shift) d = r on the FP32 path to the relative accuracy that
-    closes the gap (at most 0.5 tol |b| / |r|, with the FP32 operator's own
-    error shrinking with r) and adds d to x (at most ``max_refine`` times).
-    ``relres`` is then the FP64 residual and ``iters`` counts every FP32 CG
-    iteration.
+    CG requests on the FP32-mixed path get residual replacements (``refine``,
+    default on for FP32-mixed): the FP32 operator's rounding is relative to
+    |A| |x|, so the residual that CG's recurrence carries drifts from the
+    FP64 operator's b - (A64 + shift) x by up to ~1e-7 kappa (kappa the
+    condition number).  Each solve pauses on the device when its recurrence
+    residual claims convergence or has dropped by ``replace_ratio`` since its
+    residual was last formed; the server then forms r = b - (A64 + shift) x
+    with an FP64 handle of the same representation (h2p_hip_cg_replace) and
+    the solve goes on from there with the same search direction (the
+    reliable updates of Sleijpen and van der Vorst, 1996), or ends once that
+    residual is below tol |b|.  Unlike restarting CG on a correction
+    equation, nothing of the Krylov space is lost, so the iteration count
+    stays near the FP64 solve's.  ``relres`` is then the FP64 residual.
+    Where a solve pauses depends only on its own iterates, so its result is
+    the same whichever requests share its launches.
     """
 
     def __init__(self, hbm_budget: int, max_active: int = 64, mode: str = "jit",
                  prec: str = "fp32mix", cg_chunk: int = 8, device: str = "cuda:0",
-                 refine: bool | None = None, max_refine: int = 4, drop_host: bool = False):
+                 refine: bool | None = None, replace_ratio: float = 0.1, drop_host: bool = False):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.mode, self.prec = mode, prec
         self.refine = (prec == "fp32mix") if refine is None else bool(refine)
-        self.max_refine = int(max_refine)
+        self.replace_ratio = float(replace_ratio)
         self.drop_host = bool(drop_host)   # free a request's host representation once uploaded
         self.cg_chunk = int(cg_chunk)
         self.dev = torch.device(device)
@@ -161,21 +162,6 @@ class Server:
         self.bt_cg.close()
         self.bt_mm.close()
 
-    def _residual(self, acts):
-        """r = b - (A64 + shift) x for the listed solves on the CG stream (one
-        co-scheduled FP64 product); returns |r| / |b| per solve."""
-        torch = self.torch
-        self.bt_cg.matvec([a.mat64 for a in acts], [a.b.data_ptr() for a in acts],
-                          [a.y.data_ptr() for a in acts], [a.req.shift for a in acts],
-                          stream=self.s_cg.cuda_stream)
-        with torch.cuda.stream(self.s_cg):
-            out = []
-            for a in acts:
-                torch.sub(a.a, a.y, out=a.r)
-                out.append(torch.linalg.vector_norm(a.r) / torch.linalg.vector_norm(a.a))
-            rho = torch.stack(out).cpu().tolist()
-        return rho
-
     # ------------------------------------------------------------------
     def _admit(self, req: Request, stream):
         torch = self.torch
@@ -187,8 +173,7 @@ class Server:
         act = _Active(req, mat, a, b)
         if self.refine and req.kind == "cg":
             act.mat64 = H2MatHIP(req.h2, mode=self.mode, prec="fp64", host_blocks=False)
-            with torch.cuda.stream(stream):
-                act.r, act.d, act.y = torch.empty_like(a), torch.zeros_like(a), torch.empty_like(a)
+            mat.set_cg_replace(self.replace_ratio)
         return act
 
     def run(self, requests: list[Request], realtime: bool = True, log=None,
@@ -302,45 +287,32 @@ class Server:
             # ---- poll the solves (waits for the CG stream)
             if cg:
                 done, its, rr = self.bt_cg.cg_poll([a.mat for a in cg], stream=self.s_cg.cuda_stream)
-                keep, fin, ended = [], [], []
+                keep, fin, replace = [], [], []
                 for a, d, it, r in zip(cg, done, its, rr):
-                    if not d:
+                    a.req.iters = it
+                    if d == 0:
                         keep.append(a)
-                        continue
-                    a.req.iters += it
-                    st.cg_iters += it
-                    if a.mat64 is None:
+                    elif a.mat64 is not None:
+                        replace.append(a)      # paused, or at max_iter: its FP64 residual decides
+                    else:
                         a.req.relres = r
                         fin.append(a)
-                    else:
-                        ended.append(a)
-                # refinement: the FP64 residuals of the FP32 solves that ended
-                if ended:
-                    with torch.cuda.stream(self.s_cg):
-                        for a in ended:
-                            if a.correcting:
-                                a.b.add_(a.d)
-                    rho = self._residual(ended)
-                    again = []
-                    for a, p in zip(ended, rho):
-                        a.correcting = False
-                        if p <= a.req.tol or a.refines >= self.max_refine or not np.isfinite(p):
-                            a.req.relres = p
+                # residual replacement: r = b - (A64 + shift) x, one co-scheduled FP64 product
+                if replace:
+                    mats = [a.mat for a in replace]
+                    self.bt_cg.cg_replace(mats, [a.mat64 for a in replace], stream=self.s_cg.cuda_stream)
+                    done, its, rr = self.bt_cg.cg_poll(mats, stream=self.s_cg.cuda_stream)
+                    for a, d, it, r in zip(replace, done, its, rr):
+                        a.refines += 1
+                        a.req.iters = it
+                        if d == 1:
+                            a.req.relres = r
                             fin.append(a)
                         else:
-                            a.refines += 1
-                            a.correcting = True
-                            again.append((a, min(0.5, 0.5 * a.req.tol / p)))
                             keep.append(a)
-                    if again:
-                        with torch.cuda.stream(self.s_cg):
-                            for a, _ in again:
-                                a.d.zero_()
-                        self.bt_cg.cg_start([a.mat for a, _ in again], [a.r.data_ptr() for a, _ in again],
-                                            [a.d.data_ptr() for a, _ in again], [a.req.shift for a, _ in again],
-                                            [t for _, t in again], [a.req.max_iter for a, _ in again],
-                                            stream=self.s_cg.cuda_stream)
                 for a in fin:
+                    st.cg_iters += a.req.iters
+                    st.cg_replacements += a.refines
                     a.req.result = a.b.cpu().numpy()
                     a.req.finished = now()
                     to_free.append(a)
@@ -353,7 +325,7 @@ class Server:
                     a.mat.close()
                     if a.mat64 is not None:
                         a.mat64.close()
-                    a.a = a.b = a.r = a.d = a.y = None
+                    a.a = a.b = None
                     used -= a.req.nbytes
                 to_free = []
             # ---- only products in flight: wait for the oldest one

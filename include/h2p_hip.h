@@ -284,12 +284,31 @@ int h2p_hip_cg_start(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
                      const int *max_iter, void *stream);
 
 /* Advance every listed solve by n_iter iterations of pcg.c:63-123 (no
- * preconditioner); solves already done are left unchanged. */
+ * preconditioner); solves done or paused are left unchanged. */
 int h2p_hip_cg_step(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
                     int n_iter, void *stream);
 
-/* Wait for `stream` and report each solve's state: done flag, iterations,
- * relative residual ||r|| / ||b|| of the recurrence (pcg.c:125-127). */
+/* Residual replacement for the next h2p_hip_cg_start on h (no counterpart in
+ * the reference's pcg.c, which has one precision): with ratio > 0 the solve
+ * pauses (poll reports done = 2) when its recurrence residual claims
+ * convergence or has dropped by `ratio` since the residual was last formed
+ * as b - A x, so that the caller can form it again with a more accurate
+ * product (h2p_hip_cg_replace).  0 (the default) never pauses; ratio in
+ * [0, 1). */
+int h2p_hip_cg_set_replace(h2p_hip_handle *h, double ratio);
+
+/* Replace the residual of each listed solve: r = b - (A' + shift I) x with
+ * the product taken by hs_acc[i] (for instance an FP64 handle of the same
+ * representation as the FP32-mixed hs[i]; the same order N), then redo the
+ * stop test on it; the search direction is kept.  A paused solve resumes
+ * (done = 0) or finishes (done = 1: ||r|| <= tol ||b|| or max_iter). */
+int h2p_hip_cg_replace(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
+                       h2p_hip_handle *const *hs_acc, void *stream);
+
+/* Wait for `stream` and report each solve's state: done flag (0 running,
+ * 1 done, 2 paused for a residual replacement), iterations, relative residual
+ * ||r|| / ||b|| of the recurrence (pcg.c:125-127; after a replacement, of the
+ * replaced residual). */
 int h2p_hip_cg_poll(h2p_hip_batch *b, int n, h2p_hip_handle *const *hs,
                     int *done, int *iter, double *relres, void *stream);
 

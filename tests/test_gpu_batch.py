@@ -193,6 +193,57 @@ def test_cg_same_alone_and_in_the_mix():
         assert torch.equal(x, mixed[i][0])
 
 
+def test_cg_residual_replacement():
+    """An FP32-mixed solve with a replacement ratio pauses (done = 2) instead
+    of finishing on its recurrence residual; h2p_hip_cg_replace forms the
+    residual with the FP64 handle, and the solve ends (done = 1) only once
+    that residual is below tol.  The reported residual then matches the
+    oracle's b - (A + shift) x, and the whole run is the same whatever the
+    polling chunk."""
+    torch = _torch()
+    h2, _ = problem("matern32")
+    shift, tol, max_iter = 1e-2, 1e-8, 3000
+    m32 = H2MatHIP(h2, mode="jit", prec="fp32mix")
+    m64 = H2MatHIP(h2, mode="jit", prec="fp64")
+    bvec = np.random.default_rng(17).standard_normal(h2.krnl_mat_size)
+    bd = _dev(bvec)
+    bt = H2Batch()
+
+    def solve(chunk):
+        xd = torch.zeros_like(bd)
+        m32.set_cg_replace(0.1)
+        bt.cg_start([m32], [bd.data_ptr()], [xd.data_ptr()], [shift], [tol], [max_iter])
+        replacements = 0
+        while True:
+            bt.cg_step([m32], chunk)
+            done, its, rr = bt.cg_poll([m32])
+            if done[0] == 1:
+                break
+            if done[0] == 2:
+                bt.cg_replace([m32], [m64])
+                replacements += 1
+                done, its, rr = bt.cg_poll([m32])
+                assert done[0] in (0, 1)
+                if done[0] == 1:
+                    break
+        return xd.cpu().numpy(), its[0], rr[0], replacements
+
+    x, its, relres, nrep = solve(8)
+    true_res = rel(_shifted_oracle(h2, shift)(x), bvec)
+    print(f"[matern32 fp32mix, replacement] {its} iterations, {nrep} replacements, reported {relres:.3e}, "
+          f"oracle {true_res:.3e}", flush=True)
+    assert nrep >= 2 and its < max_iter
+    assert relres <= tol and abs(true_res - relres) <= 0.05 * tol
+    x2, its2, relres2, nrep2 = solve(3)
+    assert (its2, nrep2) == (its, nrep) and np.array_equal(x2, x) and relres2 == relres
+    m32.set_cg_replace(0.0)   # the next solve never pauses again
+    with pytest.raises(H2PHipError) as e:
+        m32.set_cg_replace(1.0)
+    assert e.value.code == abi.EINVAL
+    m32.close()
+    m64.close()
+
+
 def test_batch_rejects_bad_calls():
     torch = _torch()
     mats = _mats(MIX[:1])
@@ -204,6 +255,15 @@ def test_batch_rejects_bad_calls():
     with pytest.raises(H2PHipError) as e:
         bt.cg_step(mats, 1)
     assert e.value.code == abi.EINVAL
+    with pytest.raises(H2PHipError) as e:
+        bt.cg_replace(mats, mats)       # no solve started on it
+    assert e.value.code == abi.EINVAL
+    other = _mats([("gaussian_leaf300", "jit", "fp64")])[0]
+    bt.cg_start(mats, [x.data_ptr()], [x.data_ptr()], [1.0], [1e-6], [1])
+    with pytest.raises(H2PHipError) as e:
+        bt.cg_replace(mats, [other])    # another order
+    assert e.value.code == abi.EINVAL
+    other.close()
 
 
 def test_footprint_matches_device_bytes():

@@ -49,3 +49,33 @@ def test_request_specs_scale_with_the_rate():
     for x, y in zip(a, b):
         assert y["arrival"] == pytest.approx(x["arrival"] / 0.25, rel=1e-12)
         assert (x["N"], x["ell"], x["kind"], x["seed"]) == (y["N"], y["ell"], y["kind"], y["seed"])
+
+
+def _small_specs(monkeypatch):
+    import h2pack_amd.serve as sv
+    real = sv.request_specs
+    monkeypatch.setattr(sv, "request_specs",
+                        lambda n, seed, n_min, n_max, rate: real(n, seed, 2000, 3000, rate))
+
+
+def test_request_builds_in_worker_processes(monkeypatch):
+    """bench.py's host side of the serving legs: requests built in spawned
+    worker processes, all at once or streamed with a window and a slice."""
+    import os
+    import bench_workloads as bw
+    _small_specs(monkeypatch)
+    env = dict(os.environ)
+    try:
+        ex, futs = bw.start_request_builds(2, 7, 1.0, 50, 1)
+        reqs = bw.collect_requests(ex, futs)
+        assert [r.rid for r in reqs] == [0, 1] and all(r.h2 is not None for r in reqs)
+        assert {r.kind for r in reqs} == {"matmul", "cg"} and reqs[1].max_iter == 50
+        reqs, ex, th = bw.start_stream(4, 7, 1.0, 50, 1, window=4, first=1, count=2)
+        th.join()
+        built = [r.future.result()[0] for r in reqs]
+        ex.shutdown()
+        assert [r.rid for r in reqs] == [1, 2] and [b.rid for b in built] == [1, 2]
+        assert all(b.h2 is not None and 2000 <= b.h2.krnl_mat_size <= 3000 for b in built)
+    finally:
+        os.environ.clear()
+        os.environ.update(env)
