@@ -249,7 +249,7 @@ def main():
                    "hbm_GBps_model": gbs_model,
                    "roofline": {"bound": "hbm", "achieved": gbs_blk, "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": gbs_blk / HBM_PEAK_GBS, "traffic": None,
-                                "kernel": ("k_blocks_aot_occ<double, 5>" if tune_a["variants"][0] == 1
+                                "kernel": ("k_blocks_aot_occ<double, 6>" if tune_a["variants"][0] == 1
                                            else "k_blocks_aot<double>"), "kernel_ms": info_a["t_blocks"]},
                    "rel_err_vs_jit": float(np.linalg.norm(y_a - y_j) / np.linalg.norm(y_j)),
                    "stored_GB": blk_bytes / 1e9,

@@ -467,8 +467,24 @@ def run(args):
         rows = np.sort(rng.choice(N, 1000, replace=False))
         yref = direct_nbody(C.KRNL_RPY, [1.0], coord, np.ascontiguousarray(X[:, 0]), rows)
         Yr = Y[:, 0].reshape(N, 3)[rows].reshape(-1)
+        # single-RHS matvecs, JIT: FP64 against FP32-mixed (the tensor in FP32)
+        xv = torch.empty(Nk, dtype=torch.float64, device=dev)
+        xv.copy_(Xd[0])
+        yv = torch.empty_like(xv)
+        mv_res = {}
+        for prec in ("fp64", "fp32mix"):
+            m = H2MatHIP(h2, mode="jit", prec=prec)
+            dtm = _time(lambda: m.matvec_dev(xv.data_ptr(), yv.data_ptr(), sptr), K, W, sync)
+            mv_res[prec] = (dtm, yv.cpu().numpy())
+            m.close()
+        rpy_mv = {"fp64_ms": mv_res["fp64"][0] * 1e3, "fp32mix_ms": mv_res["fp32mix"][0] * 1e3,
+                  "fp32mix_speedup": mv_res["fp64"][0] / mv_res["fp32mix"][0],
+                  "fp32mix_rel_err_vs_fp64_gpu": _rel(mv_res["fp32mix"][1], mv_res["fp64"][1])}
+        log(f"[bench] rpy jit matvec: FP64 {rpy_mv['fp64_ms']:.2f} ms, FP32-mixed {rpy_mv['fp32mix_ms']:.2f} ms "
+            f"({rpy_mv['fp32mix_speedup']:.2f}x), rel diff {rpy_mv['fp32mix_rel_err_vs_fp64_gpu']:.2e}")
         out.update({"metric": "H2 matmuls/s (RPY N=300k, 32 RHS, FP64)", "value": res["jit"]["matmuls_per_s"],
                     "unit": "matmuls/s", "dtype": "f64", "jit": res["jit"], "aot": res["aot"],
+                    "jit_matvec": rpy_mv,
                     "rel_err_vs_direct_nbody_1000rows_col0": _rel(Yr, yref), "build_s": tb})
 
     elif args.workload == "periodic":

@@ -84,13 +84,13 @@ k_blocks_jit2_b(const BatchItem *__restrict__ items, int n, const ReqCtx *__rest
                              c.coord, c.CT, c.scale, c.diag, c.xin, c.arena);
 }
 
-template <int KID, int KD>
+template <int KID, int KD, typename T = double>
 __global__ void __launch_bounds__(256) k_blocks_jit_b(const BatchItem *__restrict__ items, int n,
                                                       const ReqCtx *__restrict__ reqs) {
     const int it = find_item(items, n, blockIdx.x);
     const ReqCtx &c = reqs[items[it].req];
     const BlkTask t = ((const BlkTask *)items[it].tasks)[blockIdx.x - items[it].begin];
-    blocks_jit_body<KID, KD>(t, c.coord, c.CT, c.kp, c.shifts, c.xin, c.arena);
+    blocks_jit_body<KID, KD, T>(t, c.coord, c.CT, c.kp, c.shifts, c.xin, c.arena);
 }
 
 template <typename TS>

@@ -126,6 +126,38 @@ __device__ __forceinline__ void rpy_coef_rcp(double dx, double dy, double dz,
     }
 }
 
+// The same in FP32 (the FP32-mixed RPY launch): 1/r from the hardware
+// reciprocal square root (within 1 ulp), the radius divisions as products
+// of the reciprocal radii.
+__device__ __forceinline__ void rpy_coef_rcp(float dx, float dy, float dz,
+                                             float ta, float sa, float ita, float isa, float C,
+                                             float &t1, float &t2,
+                                             float &ux, float &uy, float &uz) {
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const bool z = r2 == 0.0f;
+    const float inv_r = z ? 0.0f : __builtin_amdgcn_rsqf(z ? 1.0f : r2);
+    const float r = r2 * inv_r;
+    const float inv_r2 = inv_r * inv_r;
+    ux = dx * inv_r; uy = dy * inv_r; uz = dz * inv_r;
+    const float tps = ta + sa, tms = ta - sa;
+    if (r > tps) {
+        const float tmp0 = C * 0.75f * inv_r;
+        const float tmp1 = (ta * ta + sa * sa) * inv_r2;
+        t1 = tmp0 * (1.0f + tmp1 * (1.0f / 3.0f));
+        t2 = tmp0 * (1.0f - tmp1);
+    } else if (r > fabsf(tms)) {
+        const float tmp0 = tms * tms;
+        const float tmp1 = (inv_r2 * inv_r * C) * (ita * isa * (1.0f / 32.0f));
+        const float a = tmp0 + 3.0f * r2;
+        t1 = tmp1 * (16.0f * r2 * r * tps - a * a);
+        const float b = tmp0 - r2;
+        t2 = tmp1 * 3.0f * b * b;
+    } else {
+        t1 = C * fminf(ita, isa);
+        t2 = 0.0f;
+    }
+}
+
 // 1/sqrt(q) for q > 0 in FP64: the hardware estimate refined by one
 // Newton-Raphson step with its second-order term, e = 1 - q y^2,
 // y <- y + y e (1/2 + 3/8 e); the error of the estimate is cubed.

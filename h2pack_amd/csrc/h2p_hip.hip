@@ -224,8 +224,12 @@ static void free_handle(h2p_hip_handle *h) {
 
 template <int KID, int KD>
 static void launch_jit(h2p_hip_handle *h, hipStream_t s) {
-    hipLaunchKernelGGL((k_blocks_jit<KID, KD>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
-                       h->d_coord, h->CT, h->kp, h->d_shift, h->d_xin, h->d_arena);
+    if (h->prec == H2P_HIP_FP32MIX)
+        hipLaunchKernelGGL((k_blocks_jit<KID, KD, float>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                           h->d_coord, h->CT, h->kp, h->d_shift, h->d_xin, h->d_arena);
+    else
+        hipLaunchKernelGGL((k_blocks_jit<KID, KD, double>), dim3(h->n_blk), dim3(256), 0, s, h->d_blk,
+                           h->d_coord, h->CT, h->kp, h->d_shift, h->d_xin, h->d_arena);
 }
 
 template <int KID, int KD>
@@ -295,7 +299,7 @@ static void dispatch_build(h2p_hip_handle *h, hipStream_t s) {
 }
 
 // Part `part` of the grouped B/D launch of a matvec (AOT variant 1: the
-// occupancy kernels, FP64 at 5 waves per SIMD, FP32 at 8).  AOT and RPY
+// occupancy kernels, FP64 at 6 waves per SIMD, FP32 at 8).  AOT and RPY
 // handles are not split: part 0 holds every block.
 static void launch_blocks(h2p_hip_handle *h, hipStream_t s, int part) {
     if (h->part[part].n == 0) return;
@@ -313,7 +317,7 @@ static void launch_blocks(h2p_hip_handle *h, hipStream_t s, int part) {
                                h->d_arena);
     } else if (h->mode == H2P_HIP_AOT) {
         if (h->mv_var == 1)
-            hipLaunchKernelGGL((k_blocks_aot_occ<double, 5>), g, b, 0, s, h->d_blk, (const double *)h->d_blob,
+            hipLaunchKernelGGL((k_blocks_aot_occ<double, 6>), g, b, 0, s, h->d_blk, (const double *)h->d_blob,
                                h->d_xin, h->d_arena);
         else
             hipLaunchKernelGGL(k_blocks_aot<double>, g, b, 0, s, h->d_blk, (const double *)h->d_blob, h->d_xin,
@@ -702,8 +706,6 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     if (!g || !krnl) return fail(H2P_HIP_EINVAL, "NULL representation or kernel");
     if (mode != H2P_HIP_AOT && mode != H2P_HIP_JIT) return fail(H2P_HIP_EINVAL, "bad mode");
     if (prec != H2P_HIP_FP64 && prec != H2P_HIP_FP32MIX) return fail(H2P_HIP_EINVAL, "bad precision");
-    if (prec == H2P_HIP_FP32MIX && mode == H2P_HIP_JIT && krnl->id == KR)
-        return fail(H2P_HIP_ENOSYS, "FP32-mixed JIT is available for the scalar kernels");
     if (krnl->id < 0 || krnl->id > 3) return fail(H2P_HIP_EINVAL, "bad kernel id");
     const int kd = g->krnl_dim;
     if (g->pt_dim != 3) return fail(H2P_HIP_EINVAL, "pt_dim must be 3");
@@ -1459,7 +1461,9 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     h->h_mmt = std::move(mmt);
     h->h_mms = std::move(mms);
     h->h_blk = blk;
-    h->mm_ok = !(mode == H2P_HIP_AOT && prec == H2P_HIP_FP32MIX && !mm_f32);
+    // FP32-mixed handles without FP32 matmul tiles (RPY, whose matmul tiles
+    // are FP64 only) refuse matmul rather than run it in FP64
+    h->mm_ok = !(prec == H2P_HIP_FP32MIX && !mm_f32);
     if ((rc = dev_alloc(&h->d_xin, (size_t)N + S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_y1, (size_t)S, acct))) return bail(rc);
     if ((rc = dev_alloc(&h->d_yp, (size_t)S, acct))) return bail(rc);
@@ -1660,7 +1664,7 @@ static int check_matmul_args(const h2p_hip_handle *h, int layout, int n_vec, con
     if (!rm && layout != H2P_HIP_COL_MAJOR) return fail(H2P_HIP_EINVAL, "bad layout");
     const int N = h->N;
     if (rm ? (ldx < n_vec || ldy < n_vec) : (ldx < N || ldy < N)) return fail(H2P_HIP_EINVAL, "bad leading dimension");
-    if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
+    if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "FP32-mixed matmul is available for the scalar kernels in JIT mode and FP32-stored AOT blocks of them");
     return 0;
 }
 
@@ -1954,7 +1958,7 @@ int h2p_hip_autotune(h2p_hip_handle *h, int n_vec, double *t_default_ms, double 
     if (!h) return fail(H2P_HIP_EINVAL, "NULL handle");
     if (n_vec < 0) return fail(H2P_HIP_EINVAL, "n_vec must not be negative");
     if (n_vec == 0) return autotune_matvec(h, t_default_ms, t_tuned_ms);
-    if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
+    if (!h->mm_ok) return fail(H2P_HIP_ENOSYS, "FP32-mixed matmul is available for the scalar kernels in JIT mode and FP32-stored AOT blocks of them");
     const int nvp = panel_width(n_vec), cls = nv_class(nvp);
     int rc = ensure_panels(h, nvp);
     if (rc) return rc;
@@ -2310,7 +2314,8 @@ static void launch_blocks_b(const BLaunch &b, const char *dbase, const ReqCtx *r
             else hipLaunchKernelGGL((k_blocks_jit2_b<KMW, double, 4>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
         default:
-            hipLaunchKernelGGL((k_blocks_jit_b<KR, 3>), grid, blk, 0, s, it, b.n_items, reqs);
+            if (f32) hipLaunchKernelGGL((k_blocks_jit_b<KR, 3, float>), grid, blk, 0, s, it, b.n_items, reqs);
+            else hipLaunchKernelGGL((k_blocks_jit_b<KR, 3, double>), grid, blk, 0, s, it, b.n_items, reqs);
             break;
     }
 }
@@ -2487,7 +2492,7 @@ int h2p_hip_matmul_batch(h2p_hip_batch *B, int n, h2p_hip_handle *const *hs, int
         if (n_vec[i] > 0 && (!d_X[i] || !d_Y[i])) return fail(H2P_HIP_EINVAL, "NULL panel");
         if (n_vec[i] > 0 && (rm ? (ldx[i] < n_vec[i] || ldy[i] < n_vec[i]) : (ldx[i] < h->N || ldy[i] < h->N)))
             return fail(H2P_HIP_EINVAL, "bad leading dimension");
-        if (n_vec[i] > 0 && !h->mm_ok) return fail(H2P_HIP_ENOSYS, "matmul is not available for FP32-stored AOT blocks");
+        if (n_vec[i] > 0 && !h->mm_ok) return fail(H2P_HIP_ENOSYS, "FP32-mixed matmul is available for the scalar kernels in JIT mode and FP32-stored AOT blocks of them");
         max_nv = std::max(max_nv, n_vec[i]);
     }
     for (int i = 0; i < n; i++)

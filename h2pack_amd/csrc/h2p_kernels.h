@@ -455,38 +455,61 @@ __global__ void k_f64_to_f32(const double *__restrict__ in, float *__restrict__ 
 #define H2P_JIT_ROWS 256                 // target rows staged per chunk (all JIT launches)
 #define H2P_JIT_MAXNS 4
 #define H2P_RPY_MAXNS 2
-struct RpyRow { double x, y, z, a, ia, v0, v1, v2; };
+// T = double: FP64.  T = float (FP32-mixed): coordinates relative to the
+// task's first target point (the differences taken in FP64, once per point,
+// then rounded), the tensor and the products in FP32; row sums leave the
+// pass as FP32 into the FP64 partials, column sums are FP32 over 16 rows
+// per lane and folded into FP64 registers.
+template <typename T>
+struct RpyRowT { T x, y, z, a, ia, v0, v1, v2; };
+using RpyRow = RpyRowT<double>;
 
-template <int NS>
+template <int NS, typename T>
 __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
                                          const double *__restrict__ cx, long long cstride, const KParam &kp,
                                          const double *__restrict__ shifts, const double *__restrict__ xin,
-                                         double *__restrict__ arena, RpyRow *rows,
+                                         double *__restrict__ arena, RpyRowT<T> *rows,
                                          double (*red)[3][4][H2P_WAVE], bool first) {
+    constexpr bool F32 = sizeof(T) == 4;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // rows of cx: x, y, z, radius, reciprocal radius
     const double *cy = cx + cstride, *cz = cx + 2 * cstride, *ca = cx + 3 * cstride, *cia = cx + 4 * cstride;
     // set 1 may be a periodic image: its coordinates plus the task's shift
-    const double hx = shifts[3 * t.shift], hy = shifts[3 * t.shift + 1], hz = shifts[3 * t.shift + 2];
-    double sx[NS], sy[NS], sz[NS], sa[NS], isa[NS], x0[NS][3], acc1[NS][3];
+    double hx = shifts[3 * t.shift], hy = shifts[3 * t.shift + 1], hz = shifts[3 * t.shift + 2];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if constexpr (F32) {
+        ox = cx[t.c0];
+        oy = cy[t.c0];
+        oz = cz[t.c0];
+    }
+    T sx[NS], sy[NS], sz[NS], sa[NS], isa[NS], x0[NS][3], acc1[NS][3];
+    double acc1d[NS][3];
 #pragma unroll
     for (int k = 0; k < NS; k++) {
         const int p = k * H2P_WAVE + lane;
         const bool ok = p < npts;
         const int g = t.c1 + pbase + (ok ? p : 0);
-        sx[k] = cx[g] + hx;
-        sy[k] = cy[g] + hy;
-        sz[k] = cz[g] + hz;
-        sa[k] = ca[g];
-        isa[k] = cia[g];
+        if constexpr (F32) {
+            sx[k] = (T)(cx[g] + hx - ox);
+            sy[k] = (T)(cy[g] + hy - oy);
+            sz[k] = (T)(cz[g] + hz - oz);
+        } else {
+            sx[k] = cx[g] + hx;
+            sy[k] = cy[g] + hy;
+            sz[k] = cz[g] + hz;
+        }
+        sa[k] = (T)ca[g];
+        isa[k] = (T)cia[g];
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            x0[k][b] = ok ? xin[t.in0 + (pbase + p) * 3 + b] : 0.0;
-            acc1[k][b] = 0.0;
+            x0[k][b] = ok ? (T)xin[t.in0 + (pbase + p) * 3 + b] : (T)0;
+            acc1[k][b] = (T)0;
+            acc1d[k][b] = 0.0;
         }
     }
     const bool has1 = t.flags & 1;
-    const double C = kp.p1;
+    const T C = (T)kp.p1;
+    int tile = 0;
     for (int rc = 0; rc < t.n0; rc += H2P_JIT_ROWS) {
         const int nr = min(t.n0 - rc, H2P_JIT_ROWS);
         const int nr_pad = (nr + 15) & ~15;
@@ -494,39 +517,45 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
         for (int q = threadIdx.x; q < nr_pad; q += blockDim.x) {
             const bool ok = q < nr;
             const int g = t.c0 + rc + (ok ? q : 0);
-            RpyRow r;
-            r.x = cx[g];
-            r.y = cy[g];
-            r.z = cz[g];
-            r.a = ca[g];
-            r.ia = cia[g];
+            RpyRowT<T> r;
+            if constexpr (F32) {
+                r.x = (T)(cx[g] - ox);
+                r.y = (T)(cy[g] - oy);
+                r.z = (T)(cz[g] - oz);
+            } else {
+                r.x = cx[g];
+                r.y = cy[g];
+                r.z = cz[g];
+            }
+            r.a = (T)ca[g];
+            r.ia = (T)cia[g];
             const bool v = ok && has1;
-            r.v0 = v ? xin[t.in1 + (rc + q) * 3] : 0.0;
-            r.v1 = v ? xin[t.in1 + (rc + q) * 3 + 1] : 0.0;
-            r.v2 = v ? xin[t.in1 + (rc + q) * 3 + 2] : 0.0;
+            r.v0 = v ? (T)xin[t.in1 + (rc + q) * 3] : (T)0;
+            r.v1 = v ? (T)xin[t.in1 + (rc + q) * 3 + 1] : (T)0;
+            r.v2 = v ? (T)xin[t.in1 + (rc + q) * 3 + 2] : (T)0;
             rows[q] = r;
         }
         __syncthreads();
         for (int r0 = 4 * w; r0 < nr_pad; r0 += 16) {
-            double p[4][3];
+            T p[4][3];
             // one target at a time (the barriers bound the registers the
             // scheduler may spend on interleaving targets)
 #pragma unroll
             for (int a = 0; a < 4; a++) {
                 __builtin_amdgcn_sched_barrier(0);
-                const RpyRow R = rows[r0 + a];
-                p[a][0] = p[a][1] = p[a][2] = 0.0;
+                const RpyRowT<T> R = rows[r0 + a];
+                p[a][0] = p[a][1] = p[a][2] = (T)0;
 #pragma unroll
                 for (int k = 0; k < NS; k++) {
                     const bool ok = (k * H2P_WAVE + lane) < npts;
-                    double t1, t2, ux, uy, uz;
+                    T t1, t2, ux, uy, uz;
                     rpy_coef_rcp(R.x - sx[k], R.y - sy[k], R.z - sz[k], R.a, sa[k], R.ia, isa[k], C,
                                  t1, t2, ux, uy, uz);
-                    t1 = ok ? t1 : 0.0;
-                    t2 = ok ? t2 : 0.0;
+                    t1 = ok ? t1 : (T)0;
+                    t2 = ok ? t2 : (T)0;
                     // K = t1 I + t2 u u^T (symmetric)
-                    const double ud0 = ux * x0[k][0] + uy * x0[k][1] + uz * x0[k][2];
-                    const double ud1 = ux * R.v0 + uy * R.v1 + uz * R.v2;
+                    const T ud0 = ux * x0[k][0] + uy * x0[k][1] + uz * x0[k][2];
+                    const T ud1 = ux * R.v0 + uy * R.v1 + uz * R.v2;
                     p[a][0] += t1 * x0[k][0] + t2 * ux * ud0;
                     p[a][1] += t1 * x0[k][1] + t2 * uy * ud0;
                     p[a][2] += t1 * x0[k][2] + t2 * uz * ud0;
@@ -539,11 +568,23 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
             const int myrow = r0 + ((lane >> 4) & 3);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const double s = wave_sum4<double>(p[0][c], p[1][c], p[2][c], p[3][c]);
+                const T s = wave_sum4<T>(p[0][c], p[1][c], p[2][c], p[3][c]);
                 if ((lane & 15) == 0 && myrow < nr) {
                     double *o = arena + t.out0 + (rc + myrow) * 3 + c;
-                    if (first) *o = s;
-                    else *o += s;
+                    if (first) *o = (double)s;
+                    else *o += (double)s;
+                }
+            }
+            if constexpr (F32) {
+                if (++tile == 4) {
+                    tile = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; k++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) {
+                            acc1d[k][b] += (double)acc1[k][b];
+                            acc1[k][b] = (T)0;
+                        }
                 }
             }
         }
@@ -553,7 +594,10 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
 #pragma unroll
         for (int k = 0; k < NS; k++)
 #pragma unroll
-            for (int b = 0; b < 3; b++) red[k][b][w][lane] = acc1[k][b];
+            for (int b = 0; b < 3; b++) {
+                if constexpr (F32) red[k][b][w][lane] = acc1d[k][b] + (double)acc1[k][b];
+                else red[k][b][w][lane] = acc1[k][b];
+            }
         __syncthreads();
         for (int q = threadIdx.x; q < npts * 3; q += blockDim.x) {
             const int pnt = q / 3, b = q - pnt * 3;
@@ -564,32 +608,32 @@ __device__ __forceinline__ void rpy_pass(const BlkTask &t, int pbase, int npts,
     }
 }
 
-template <int KID, int KD>
+template <int KID, int KD, typename T = double>
 __device__ __forceinline__ void blocks_jit_body(const BlkTask &t, const double *__restrict__ cx,
                                                 long long cstride, const KParam &kp,
                                                 const double *__restrict__ shifts,
                                                 const double *__restrict__ xin,
                                                 double *__restrict__ arena) {
     static_assert(KID == KR && KD == 3, "the per-pass launch is the RPY kernel's");
-    __shared__ RpyRow rows[H2P_JIT_ROWS];
+    __shared__ RpyRowT<T> rows[H2P_JIT_ROWS];
     __shared__ double red[H2P_RPY_MAXNS][3][4][H2P_WAVE];
     for (int pb = 0; pb < t.n1; pb += H2P_RPY_MAXNS * H2P_WAVE) {
         const int np = min(t.n1 - pb, H2P_RPY_MAXNS * H2P_WAVE);
         const bool first = (pb == 0);
-        if (np <= H2P_WAVE) rpy_pass<1>(t, pb, np, cx, cstride, kp, shifts, xin, arena, rows, red, first);
-        else rpy_pass<2>(t, pb, np, cx, cstride, kp, shifts, xin, arena, rows, red, first);
+        if (np <= H2P_WAVE) rpy_pass<1, T>(t, pb, np, cx, cstride, kp, shifts, xin, arena, rows, red, first);
+        else rpy_pass<2, T>(t, pb, np, cx, cstride, kp, shifts, xin, arena, rows, red, first);
         __syncthreads();
     }
 }
 
-template <int KID, int KD>
+template <int KID, int KD, typename T = double>
 __global__ void __launch_bounds__(256) k_blocks_jit(const BlkTask *__restrict__ tasks,
                                                     const double *__restrict__ cx,
                                                     long long cstride, KParam kp,
                                                     const double *__restrict__ shifts,
                                                     const double *__restrict__ xin,
                                                     double *__restrict__ arena) {
-    blocks_jit_body<KID, KD>(tasks[blockIdx.x], cx, cstride, kp, shifts, xin, arena);
+    blocks_jit_body<KID, KD, T>(tasks[blockIdx.x], cx, cstride, kp, shifts, xin, arena);
 }
 
 // The root's periodic block (H2P_matvec_periodic, src/H2Pack_matvec_periodic.c:

@@ -48,11 +48,11 @@ extern "C" {
 #define H2P_HIP_FP32MIX 1  /* FP32 kernel values (JIT) or FP32-stored blocks
                               (AOT) and FP32 B/D products, their FP32 partial
                               sums folded into FP64; U sweeps and the output
-                              accumulation stay FP64.  Matvec: all kernels in
-                              AOT, the scalar kernels in JIT (RPY JIT:
-                              H2P_HIP_ENOSYS at upload).  Matmul: the scalar
-                              kernels, JIT and AOT, on FP32 matrix cores
-                              (RPY AOT: H2P_HIP_ENOSYS). */
+                              accumulation stay FP64.  Matvec: all kernels,
+                              JIT and AOT (periodic handles: FP64 only).
+                              Matmul: the scalar kernels, JIT and AOT, on
+                              FP32 matrix cores (RPY: H2P_HIP_ENOSYS, there
+                              are no FP32 RPY matmul tiles). */
 
 /* CBLAS_LAYOUT values */
 #define H2P_HIP_ROW_MAJOR 101

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 # (problem, mode, precision) of the matrices mixed in one batch
 MIX = [("gaussian", "jit", "fp64"), ("matern32", "jit", "fp32mix"), ("coulomb", "aot", "fp64"),
-       ("rpy", "jit", "fp64"), ("coulomb_diag", "aot", "fp32mix")]
+       ("rpy", "jit", "fp64"), ("coulomb_diag", "aot", "fp32mix"), ("rpy", "jit", "fp32mix")]
 
 
 def _torch():
@@ -88,7 +88,7 @@ def test_result_is_the_same_alone_and_in_the_mix():
         b.matvec([m], [x.data_ptr()], [y.data_ptr()], [0.5])
         alone.append(y)
     mixed = [torch.empty_like(x) for x in xs]
-    order = [3, 0, 4, 1, 2]
+    order = [3, 5, 0, 4, 1, 2]
     b.matvec([mats[i] for i in order], [xs[i].data_ptr() for i in order],
              [mixed[i].data_ptr() for i in order], [0.5] * len(order))
     torch.cuda.synchronize()
