@@ -1812,46 +1812,22 @@ int h2p_hip_matmul(h2p_hip_handle *h, int layout, int n_vec, const double *X, in
     return rc;
 }
 
-// Mean time of the matvec interior (replays of a graph of it captured for
-// the timing, or eager launches with graphs off) over reps calls after one
-// warm-up, on the handle's stream.
+// Mean time of the matvec interior over reps calls after one warm-up,
+// launched eagerly on the handle's streams: the same kernels, streams and
+// events as the captured matvec graph, without instantiating a graph per
+// schedule (each instantiate / launch / destroy cycle of a forked graph is
+// one more pass through the runtime's graph launch path, where processes
+// have taken host faults).
 static int time_interior(h2p_hip_handle *h, int reps, double *ms) {
     hipStream_t s = h->own_stream;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    if (h->use_graph) {
-        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        launch_interior(h, s, nullptr);
-        HIP_TRY(hipStreamEndCapture(s, &g));
-        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            return fail(H2P_HIP_EDEVICE, "hipGraphInstantiate failed");
-        }
-    }
-    struct Drop {
-        hipGraph_t g;
-        hipGraphExec_t e;
-        hipStream_t s;
-        ~Drop() {
-            if (!e) return;
-            (void)hipStreamSynchronize(s);
-            (void)hipGraphExecDestroy(e);
-            (void)hipGraphDestroy(g);
-        }
-    } drop{g, ge, s};
-    auto once = [&]() -> int {
-        if (ge) HIP_TRY(hipGraphLaunch(ge, s));
-        else launch_interior(h, s, nullptr);
-        return 0;
-    };
-    int rc = once();
-    if (rc) return rc;
+    launch_interior(h, s, nullptr);   // warm-up
+    HIP_TRY(hipGetLastError());
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float t = 0.f;
-    rc = hipEventRecord(e0, s) == hipSuccess ? 0 : fail(H2P_HIP_EDEVICE, "hipEventRecord failed");
-    for (int r = 0; r < reps && !rc; r++) rc = once();
+    int rc = hipEventRecord(e0, s) == hipSuccess ? 0 : fail(H2P_HIP_EDEVICE, "hipEventRecord failed");
+    for (int r = 0; r < reps && !rc; r++) launch_interior(h, s, nullptr);
     if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&t, e0, e1) != hipSuccess))
         rc = fail(H2P_HIP_EDEVICE, "timing the matvec interior failed");
