@@ -348,6 +348,9 @@ def main():
                                   "achieved_TFLOPs": flops_d / (info_j["t_blocks_d"] * 1e-3) / 1e12},
                      "traffic_source": "profiles/gaussian_1M_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "note": "achieved = H2Pack JIT_flops model (14 flops per Gaussian pair) / kernel time; "
+                             "the B blocks are two launches of this kernel per matvec (the deep levels' blocks "
+                             "and, with the level split, the shallower levels'), and kernel_ms is their sum "
+                             "measured with HIP events in the profiled pass, where they run in order; "
                              "FP64 VALU and FP64 MFMA share the 78.6 TFLOP/s peak (one FP64 pipe: "
                              "SQ_VALU_MFMA_COEXEC_CYCLES = 0); evaluating a pair issues ~18 pipe slots "
                              "(2^x 12 FP64/integer operations with its 128-entry LDS table, the squared "
