@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: most requests on the GPU at once")
     ap.add_argument("--cg-max-iter", type=int, default=3000, help="serving: CG iteration cap")
     ap.add_argument("--build-workers", type=int, default=4, help="serving: host construction processes")
+    ap.add_argument("--cg-active", type=int, default=0,
+                    help="serving, one leg: most CG solves advanced per tick, the longest admitted (0: all)")
     ap.add_argument("--serve-legs", default="both", choices=["both", "batched", "alone", "alone-service"],
                     help="serving: both legs on prebuilt requests, one leg on requests built while served, "
                          "or the FP64-alone service time of each request (back to back, no arrival clock)")

@@ -143,13 +143,14 @@ def collect_requests(ex, futs):
     return reqs
 
 
-def serve_one(reqs, name, prec, slots, hbm_frac=0.8, drop_host=False):
+def serve_one(reqs, name, prec, slots, hbm_frac=0.8, drop_host=False, cg_active=None):
     """One serving leg over ``reqs``; returns its summary dict."""
     import torch
     from h2pack_amd.serve import Server
     free, total = torch.cuda.mem_get_info()
     budget = int(hbm_frac * free)
-    srv = Server(hbm_budget=budget, max_active=slots, prec=prec, cg_chunk=8, drop_host=drop_host)
+    srv = Server(hbm_budget=budget, max_active=slots, prec=prec, cg_chunk=8, drop_host=drop_host,
+                 cg_active=cg_active)
     log(f"[bench] serving {len(reqs)} requests: {name} ({prec}, up to {slots} at once)")
     st = srv.run(reqs, realtime=True, log=log)
     srv.close()
@@ -313,7 +314,8 @@ def run_serving(args):
                 log(f"[bench] head start: {done}/{head} built after {t_log - t0:.0f}s")
             time.sleep(0.2)
         log(f"[bench] head start: {head} requests built in {time.perf_counter() - t0:.1f}s")
-        d = serve_one(reqs, leg[0], leg[1], leg[2], drop_host=True)
+        d = serve_one(reqs, leg[0], leg[1], leg[2], drop_host=True,
+                      cg_active=args.cg_active if args.cg_active > 0 else None)
         th.join()
         ex.shutdown()
         out = {"metric": "served GP requests/s and p50/p99 latency (Matern-3/2)", "value": d["throughput"],
@@ -322,6 +324,7 @@ def run_serving(args):
                "config": {"workload": f"serving_matern32_{args.requests}req_N1e4-4e5_density10_reltol1e-6_leaf300"
                                       f"_rate{args.rate}_8rhs_cg_tol1e-6_nugget1e-2_{leg[0]}",
                           "requests": args.requests, "rate_per_s": args.rate, "slots": leg[2],
+                          "cg_active": args.cg_active if args.cg_active > 0 else None,
                           "cg_max_iter": args.cg_max_iter, "build_workers": args.build_workers},
                leg[0]: d, "vs_baseline": None}
         print(json.dumps(out), flush=True)

@@ -141,7 +141,8 @@ class Server:
 
     def __init__(self, hbm_budget: int, max_active: int = 64, mode: str = "jit",
                  prec: str = "fp32mix", cg_chunk: int = 8, device: str = "cuda:0",
-                 refine: bool | None = None, replace_ratio: float = 0.1, drop_host: bool = False):
+                 refine: bool | None = None, replace_ratio: float = 0.1, drop_host: bool = False,
+                 cg_active: int | None = None):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
@@ -149,6 +150,9 @@ class Server:
         self.mode, self.prec = mode, prec
         self.refine = (prec == "fp32mix") if refine is None else bool(refine)
         self.replace_ratio = float(replace_ratio)
+        # at most this many CG solves (the longest admitted) advance per tick;
+        # the others stay resident and wait (None: all of them)
+        self.cg_active = None if cg_active is None else max(1, int(cg_active))
         self.drop_host = bool(drop_host)   # free a request's host representation once uploaded
         self.cg_chunk = int(cg_chunk)
         self.dev = torch.device(device)
@@ -270,7 +274,9 @@ class Server:
                                     stream=self.s_cg.cuda_stream)
                 cg.extend(new_cg)
             if cg:
-                self.bt_cg.cg_step([a.mat for a in cg], self.cg_chunk, stream=self.s_cg.cuda_stream)
+                run = cg if self.cg_active is None else \
+                    sorted(cg, key=lambda a: (a.req.admitted, a.req.rid))[:self.cg_active]
+                self.bt_cg.cg_step([a.mat for a in run], self.cg_chunk, stream=self.s_cg.cuda_stream)
                 st.ticks += 1
             # ---- retire finished products
             still = []

@@ -48,6 +48,19 @@ def test_serving_same_alone_and_in_the_mix(reqs):
         assert alone[0].iters == m.iters
 
 
+def test_serving_cap_on_stepped_solves_keeps_the_results(reqs):
+    """Advancing only the longest-admitted solves each tick (cg_active)
+    changes when solves run, not what they compute."""
+    full = _clone(reqs)
+    Server(hbm_budget=64 << 30, max_active=8, prec="fp32mix", cg_chunk=4).run(full, realtime=False)
+    capped = _clone(reqs)
+    st = Server(hbm_budget=64 << 30, max_active=8, prec="fp32mix", cg_chunk=4, cg_active=1).run(
+        capped, realtime=False)
+    assert st.n_done == len(reqs)
+    for a, b in zip(full, capped):
+        assert np.array_equal(a.result, b.result) and a.iters == b.iters
+
+
 def test_serving_fp32mix_against_fp64(reqs):
     mix = _clone(reqs)
     Server(hbm_budget=64 << 30, max_active=8, prec="fp32mix").run(mix, realtime=False)
