@@ -81,8 +81,8 @@ def main():
     ap.add_argument("--slots", type=int, default=32, help="serving: most requests on the GPU at once")
     ap.add_argument("--cg-max-iter", type=int, default=3000, help="serving: CG iteration cap")
     ap.add_argument("--build-workers", type=int, default=4, help="serving: host construction processes")
-    ap.add_argument("--cg-active", type=int, default=0,
-                    help="serving, one leg: most CG solves advanced per tick, the longest admitted (0: all)")
+    ap.add_argument("--cg-active", type=int, default=16,
+                    help="serving: most CG solves advanced per tick, the longest admitted first (0: all)")
     ap.add_argument("--serve-legs", default="both", choices=["both", "batched", "alone", "alone-service"],
                     help="serving: both legs on prebuilt requests, one leg on requests built while served, "
                          "or the FP64-alone service time of each request (back to back, no arrival clock)")
@@ -271,12 +271,13 @@ def main():
         try:
             reqs = bench_workloads.collect_requests(*serve_builds)
             t_sb = time.perf_counter() - t_sb
-            r = bench_workloads.serve_compare(reqs, slots=args.serve_requests)
+            cg_active = args.cg_active if args.cg_active > 0 else None
+            r = bench_workloads.serve_compare(reqs, slots=args.serve_requests, cg_active=cg_active)
             d32, d64 = r["batched_fp32mix"], r["alone_fp64"]
             serving = {"workload": f"serving_matern32_{args.serve_requests}req_N1e4-4e5_density10_reltol1e-6_"
                                    f"leaf300_poisson{args.serve_rate}_8rhs_cg_tol1e-6_nugget1e-2",
                        "requests": args.serve_requests, "rate_per_s": args.serve_rate,
-                       "p50_s": d32["p50"], "p99_s": d32["p99"], "throughput_req_per_s": d32["throughput"],
+                       "cg_active": cg_active, "p50_s": d32["p50"], "p99_s": d32["p99"], "throughput_req_per_s": d32["throughput"],
                        "alone_fp64_p50_s": d64["p50"], "alone_fp64_p99_s": d64["p99"],
                        "alone_fp64_throughput_req_per_s": d64["throughput"],
                        "host_build_s_overlapped": t_sb, **r}

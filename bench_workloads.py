@@ -170,7 +170,7 @@ def serve_one(reqs, name, prec, slots, hbm_frac=0.8, drop_host=False, cg_active=
     return d
 
 
-def serve_compare(reqs, slots, hbm_frac=0.8):
+def serve_compare(reqs, slots, hbm_frac=0.8, cg_active=16):
     """The requests through the continuous-batching loop (FP32-mixed JIT,
     co-scheduled launches) and, on the same arrivals, one at a time on the
     plain FP64 path; returns the summary dict."""
@@ -191,7 +191,7 @@ def serve_compare(reqs, slots, hbm_frac=0.8):
     res = {}
     for name, prec, sl in (("batched_fp32mix", "fp32mix", slots), ("alone_fp64", "fp64", 1)):
         rs = fresh()
-        srv = Server(hbm_budget=budget, max_active=sl, prec=prec, cg_chunk=8)
+        srv = Server(hbm_budget=budget, max_active=sl, prec=prec, cg_chunk=8, cg_active=cg_active)
         log(f"[bench] serving {len(rs)} requests: {name} ({prec}, up to {sl} at once)")
         st = srv.run(rs, realtime=True, log=log)
         srv.close()
@@ -333,7 +333,7 @@ def run_serving(args):
     ex, futs = start_request_builds(args.requests, args.seed, args.rate, args.cg_max_iter, args.build_workers)
     reqs = collect_requests(ex, futs)
     t_build = time.perf_counter() - t0
-    r = serve_compare(reqs, args.slots)
+    r = serve_compare(reqs, args.slots, cg_active=args.cg_active if args.cg_active > 0 else None)
     d32 = r["batched_fp32mix"]
     out = {"metric": "served GP requests/s and p50/p99 latency (Matern-3/2, FP32-mixed batched vs FP64 alone)",
            "value": d32["throughput"], "unit": "requests/s", "higher_is_better": True,
