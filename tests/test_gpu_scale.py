@@ -40,6 +40,36 @@ def test_large_matvec_matches_oracle(kid, param, n):
         m.close()
 
 
+def test_rpy_at_the_matmul_configuration_shape():
+    """RPY (radius 1, 20% volume fraction, reltol 1e-7, leaf 200: the shape
+    of the RPY matmul configuration at N = 30k): matvec in JIT FP64, AOT
+    FP64 and JIT FP32-mixed, and an 8-column matmul in JIT and AOT, against
+    the oracle."""
+    import math
+    n = 30_000
+    L = (n * (4.0 / 3.0) * math.pi / 0.2) ** (1.0 / 3.0)
+    rng = np.random.default_rng(19)
+    coord = np.concatenate([rng.random((3, n)) * L, np.ones((1, n))], axis=0)
+    h2 = C.build_h2(coord, C.KRNL_RPY, [1.0], 1e-7, 200, BD_JIT=1, pp_kind="surface", max_leaf_size=4.0,
+                    host_blocks=False)
+    orc = O.OracleH2(h2, BD_JIT=1)
+    x = rng.random(h2.krnl_mat_size) - 0.5
+    X = rng.standard_normal((h2.krnl_mat_size, 8))
+    y_ref = orc.matvec(x, n_thread=16)
+    Y_ref = orc.matmul(X, "col", n_thread=16)
+    for mode, prec, bar in (("jit", "fp64", 1e-12), ("aot", "fp64", 1e-12), ("jit", "fp32mix", 1e-6)):
+        m = H2MatHIP(h2, mode=mode, prec=prec, host_blocks=False)
+        err = rel(m.matvec(x), y_ref)
+        msg = f"[rpy N={n} {mode} {prec}] matvec rel err vs oracle {err:.3e}"
+        if prec == "fp64":
+            err_mm = rel(m.matmul(X, "col"), Y_ref)
+            msg += f", 8-column matmul {err_mm:.3e}"
+            assert err_mm < bar
+        print(msg, flush=True)
+        assert err < bar
+        m.close()
+
+
 def test_large_matmul_matches_oracle():
     h2 = _build(C.KRNL_MATERN32, [1.0], 100_000, 300, 18)
     X = np.random.default_rng(5).standard_normal((h2.krnl_mat_size, 8))
