@@ -296,7 +296,8 @@ def main():
                          f"{el:.1f}s, oracle/h2oracle.c (OpenMP, no BLAS)"}
         log(f"[bench] oracle CPU: {rate:.3f} matvec/s on {nt} threads; GPU vs oracle rel err {err_oracle:.3e}")
 
-    # HBM bytes per launch of the named kernel, from the committed rocprofv3
+    # HBM bytes of the named kernel in one matvec (its launches summed where it
+    # runs as two, like the JIT block kernels), from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this workload (null when absent)
     def traffic(kernel):
         try:
@@ -304,7 +305,8 @@ def main():
                 t = json.load(f)
             if N != 1_000_000:
                 return None
-            return t["kernels"][kernel]["hbm_bytes"]
+            e = t["kernels"][kernel]
+            return e.get("hbm_bytes_per_matvec", e["hbm_bytes"])
         except (OSError, KeyError, ValueError):
             return None
 
