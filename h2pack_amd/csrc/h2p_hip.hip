@@ -855,6 +855,18 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
     std::vector<FwdTask> fwdt;
     for (int lv = g->max_level; lv >= g->min_adm_level; lv--) {
         Level L{(int)fwdt.size(), 0};
+        // column chunk of the level's tasks: 256 columns, narrower while the
+        // level would have fewer than H2P_FWD_MIN_WG workgroups (each
+        // workgroup keeps the same loads in flight at any width, so more of
+        // them on a small level means more bytes in flight)
+        int ncol = 4 * H2P_WAVE;
+        while (ncol > H2P_WAVE) {
+            long long nwg = 0;
+            for (int j = 0; j < g->level_n_node[lv]; j++)
+                nwg += (g->U_ncol[g->level_nodes[(size_t)lv * nlf + j]] + ncol - 1) / ncol;
+            if (nwg >= H2P_FWD_MIN_WG) break;
+            ncol >>= 1;
+        }
         for (int j = 0; j < g->level_n_node[lv]; j++) {
             int n = g->level_nodes[(size_t)lv * nlf + j];
             if (g->U_ncol[n] == 0) continue;
@@ -872,7 +884,8 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
                 if (sum != t.nrow) return bail(fail(H2P_HIP_EINVAL, "non-leaf U_nrow != sum of child ranks"));
                 t.in_off = N + soff[g->children[(size_t)n * mc]];
             }
-            for (int c0 = 0; c0 < t.r; c0 += 4 * H2P_WAVE) {
+            t.ncol = ncol;
+            for (int c0 = 0; c0 < t.r; c0 += ncol) {
                 t.col0 = c0;
                 fwdt.push_back(t);
             }
@@ -1297,7 +1310,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
         auto in = [](long long off, long long len, long long cap) { return off >= 0 && len >= 0 && off + len <= cap; };
         for (const FwdTask &t : fwdt)
             if (!in(t.in_off, t.nrow, XIN) || !in(t.out_off, t.r, XIN) || !in(t.u_off, (long long)t.nrow * t.r, utot) ||
-                t.col0 < 0 || t.col0 >= t.r)
+                t.col0 < 0 || t.col0 >= t.r || (t.ncol != 64 && t.ncol != 128 && t.ncol != 256))
                 return bail(fail(H2P_HIP_EINVAL, "upward task out of range"));
         for (const BwdTask2 &t : bwdt)
             if (!in(t.y1_off, t.r, Y1) || !in(t.out_off, t.nrow, Y1) || !in(t.ut_off, (long long)t.nrow * t.r, utot) ||
@@ -1371,7 +1384,7 @@ static int upload_impl(const h2p_hip_h2pack *g, const h2p_kernel_desc *krnl, int
             const Level &L = h->fwd_levels[k];
             for (int i = L.off; i < L.off + L.n; i++) {
                 const FwdTask &t = fwdt[i];
-                for (int c = t.col0; c < std::min(t.r, t.col0 + 4 * H2P_WAVE); c++)
+                for (int c = t.col0; c < std::min(t.r, t.col0 + t.ncol); c++)
                     y0c[(size_t)(t.out_off - N + c)] = k < fd ? 1 : 2;
             }
         }

@@ -28,11 +28,12 @@ struct BlkTask {
 };
 
 // Upward-sweep task: y0[out] = U^T in, U row-major nrow x r.
+#define H2P_FWD_MIN_WG 256    // a level's upward tasks narrow their column chunks to reach this many (one per CU)
 struct FwdTask {
     long long u_off;
     int nrow, r, in_off, out_off;
-    int col0;         // first of the (at most 4 x 64) output columns this task computes
-    int pad;
+    int col0;         // first of the output columns this task computes
+    int ncol;         // at most this many columns from col0: 64, 128 or 256
 };
 
 // Segmented reduction into y1: y1[dst + e] = sum_k arena[contrib[k] + e].
@@ -85,15 +86,36 @@ __global__ void __launch_bounds__(256) k_reduce_y1(const RedTask *__restrict__ t
 // Sweep kernels.
 //
 // Upward sweep (src/H2Pack_matvec.c:95-177): one workgroup per node and
-// 256-column chunk of its U (so a node with a large basis spreads over
-// several workgroups); lanes own the chunk's columns (NS strips of 64, only
-// the strips that hold columns), the four waves take contiguous quarters of
-// the rows and load 8 rows x NS strips before their FMAs (rows in order, so
-// each column's sum runs over the rows in order whatever the unrolling); the
-// four wave partials are added in LDS in a fixed order.
+// column chunk of its U (256 columns, or 128 / 64 on a level with fewer
+// workgroups than CUs, so a node with a large basis spreads over several
+// workgroups); lanes own the chunk's columns (NS strips of 64, only the
+// strips that hold columns), the four waves take contiguous quarters of the
+// rows and load UR = 32 / NS rows x NS strips before their FMAs (32 loads
+// in flight per wave whatever the chunk width, 24 at NS = 3; then 8-row
+// batches, then single rows; rows in order, so each column's sum runs over
+// the rows in order whatever the unrolling and the chunk width); the four
+// wave partials are added in LDS in a fixed order.
+// R rows x NS strips loaded, then their FMAs in row order.
+template <int R, int NS>
+__device__ __forceinline__ void fwd2_rows(const double *__restrict__ Ut, int ld, const double *__restrict__ in, int i,
+                                          const int *cc, double *acc) {
+    double u[R][NS];
+#pragma unroll
+    for (int a = 0; a < R; a++)
+#pragma unroll
+        for (int k = 0; k < NS; k++) u[a][k] = Ut[(long long)(i + a) * ld + cc[k]];
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        const double v = in[i + a];
+#pragma unroll
+        for (int k = 0; k < NS; k++) acc[k] += u[a][k] * v;
+    }
+}
+
 template <int NS>
 __device__ __forceinline__ void fwd2_strips(const FwdTask &t, const double *__restrict__ U,
                                             double *__restrict__ xin, double (*red)[4][H2P_WAVE]) {
+    constexpr int UR = NS == 3 ? 8 : 32 / NS;
     // the wave index is the same on every lane: said so, the row addresses
     // stay scalar and each load needs only its column offset per lane
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -113,19 +135,9 @@ __device__ __forceinline__ void fwd2_strips(const FwdTask &t, const double *__re
         cc[k] = ok[k] ? cc[k] : 0;
     }
     int i = i0;
-    for (; i + 7 < i1; i += 8) {
-        double u[8][NS];
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int k = 0; k < NS; k++) u[a][k] = Ut[(long long)(i + a) * t.r + cc[k]];
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            const double v = in[i + a];
-#pragma unroll
-            for (int k = 0; k < NS; k++) acc[k] += u[a][k] * v;
-        }
-    }
+    for (; i + UR - 1 < i1; i += UR) fwd2_rows<UR, NS>(Ut, t.r, in, i, cc, acc);
+    if (UR > 8)
+        for (; i + 7 < i1; i += 8) fwd2_rows<8, NS>(Ut, t.r, in, i, cc, acc);
     for (; i < i1; i++) {
         const double v = in[i];
 #pragma unroll
@@ -146,7 +158,7 @@ __device__ __forceinline__ void fwd2_strips(const FwdTask &t, const double *__re
 __device__ __forceinline__ void fwd2_body(const FwdTask &t, const double *__restrict__ U,
                                           double *__restrict__ xin) {
     __shared__ double red[4][4][H2P_WAVE];
-    const int ns = (min(t.r - t.col0, 4 * H2P_WAVE) + H2P_WAVE - 1) / H2P_WAVE;
+    const int ns = (min(t.r - t.col0, t.ncol) + H2P_WAVE - 1) / H2P_WAVE;
     if (ns <= 1) fwd2_strips<1>(t, U, xin, red);
     else if (ns == 2) fwd2_strips<2>(t, U, xin, red);
     else if (ns == 3) fwd2_strips<3>(t, U, xin, red);
